@@ -1,0 +1,216 @@
+/*
+ * ifl.h -- C ABI of the MI355X-native incremental-fluids solver step.
+ *
+ * The reference (tunabrain/incremental-fluids) is a set of single-file C++
+ * programs whose only interface is the FluidSolver / FluidQuantity /
+ * SolidBody class API. This header is the flat C boundary under a C++ shim
+ * (ifl_fluid.hpp) that keeps those class names, constructors and update().
+ * Every entry point below cites the reference member it stands in for
+ * (N:line = N-<solver>/Fluid.cpp:line in the reference tree).
+ *
+ * Conventions: plain pointers and sizes, int status return (IFL_OK == 0),
+ * nothing throws across the boundary, device memory never escapes.
+ * Field layout is the reference's own row-major x + y*w per quantity.
+ */
+#ifndef IFL_H
+#define IFL_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define IFL_ABI_VERSION 1
+
+/* ---- status codes (the reference has none: new throws, PCG printf's) ---- */
+enum {
+    IFL_OK = 0,
+    IFL_ERR_ARG = 1,          /* bad argument (size, id, level, pointer) */
+    IFL_ERR_ALLOC = 2,        /* device allocation failed */
+    IFL_ERR_HIP = 3,          /* a HIP runtime call failed */
+    IFL_ERR_UNSUPPORTED = 4,  /* operation not available at this level */
+    IFL_ERR_NO_DEVICE = 5,    /* no usable gfx950 device */
+    IFL_ERR_DEVICE = 6        /* a kernel reported a bounded-wait timeout */
+};
+
+/* ---- cell types, 7:104-107 and 8:115-119 ---- */
+enum { IFL_CELL_FLUID = 0, IFL_CELL_SOLID = 1, IFL_CELL_EMPTY = 2 };
+
+/* ---- solid bodies, SolidBox 7:169-218 / SolidSphere 7:220-257 ---- */
+enum { IFL_BODY_BOX = 0, IFL_BODY_SPHERE = 1 };
+
+typedef struct ifl_body {
+    int32_t type;          /* IFL_BODY_BOX or IFL_BODY_SPHERE */
+    int32_t reserved;
+    double pos_x, pos_y;   /* SolidBody::_posX/_posY, 7:111-112 */
+    double scale_x, scale_y;
+    double theta;
+    double vel_x, vel_y, vel_theta;
+    /* rotate(x, y, phi) (7:54-58) evaluates libm cos/sin; the caller passes
+     * the host values so device results match the host libm bit for bit. */
+    double cos_neg, sin_neg;   /* cos(-theta), sin(-theta) */
+    double cos_pos, sin_pos;   /* cos(theta),  sin(theta)  */
+} ifl_body;
+
+/* ---- configuration ---- */
+enum { IFL_PREC_FP64 = 0, IFL_PREC_MIXED = 1 };
+enum { IFL_DOT_TREE = 0, IFL_DOT_SEQUENTIAL = 1 };
+
+typedef struct ifl_config {
+    int32_t level;        /* 3..8: which reference solver's update() to run */
+    int32_t w, h;         /* grid size in cells */
+    int32_t precision;    /* IFL_PREC_FP64 (drop-in) or IFL_PREC_MIXED */
+    int32_t dot_order;    /* IFL_DOT_TREE (fast) or IFL_DOT_SEQUENTIAL
+                             (reference summation order, for parity runs) */
+    int32_t pcg_limit;    /* project(limit): 600 at level 3 (3:437), else 2000 (7:1016) */
+    int32_t verbose;      /* 1: print the reference's stdout lines per solve */
+    int32_t device;       /* HIP device ordinal */
+    double density;       /* level 3-5 fluid density; 6-8 air density (7:605) */
+    double density_soot;  /* level 6-8 (7:606) */
+    double diffusion;     /* level 6-8 (7:607) */
+    double tau, sigma;    /* MIC(0) constants 0.97 / 0.25 (7:740-741) */
+    double tol;           /* PCG absolute tolerance on the max residual, 1e-5 (7:859) */
+    double t_amb;         /* 294 (7:950) */
+    double gravity;       /* 9.81 (7:951) */
+    double flip_alpha;    /* 0.001 (8:1296) */
+    double mixed_inner_rtol; /* mixed form: inner FP32 PCG relative reduction per refinement */
+    int32_t mixed_max_outer; /* mixed form: refinement sweeps before giving up */
+    int32_t reserved[7];
+} ifl_config;
+
+/* Fills cfg with the reference's literal defaults for `level`. */
+int ifl_config_default(ifl_config* cfg, int level, int w, int h);
+
+/* ---- statistics returned in place of the reference's printf lines ---- */
+enum {
+    IFL_SOLVE_CONVERGED = 0,      /* "Exiting solver after %d iterations ..." 7:874 */
+    IFL_SOLVE_INITIAL_SMALL = 1,  /* "Initial guess sufficiently small" 7:860 */
+    IFL_SOLVE_BUDGET = 2,         /* "Exceeded budget of %d iterations ..." 7:885 */
+    IFL_SOLVE_NOT_RUN = 3
+};
+
+typedef struct ifl_solve_stats {
+    int32_t status;       /* IFL_SOLVE_* */
+    int32_t iterations;   /* the `iter` the reference prints on convergence,
+                             or the limit on budget exhaustion */
+    double max_error;     /* final max |r| over the reference's norm domain */
+    int32_t outer;        /* mixed form: refinement sweeps; 0 for FP64 */
+    int32_t inner_total;  /* mixed form: summed inner iterations */
+} ifl_solve_stats;
+
+typedef struct ifl_step_stats {
+    ifl_solve_stats heat;      /* levels 6-8 (7:1001-1005) */
+    ifl_solve_stats pressure;  /* all levels */
+    int32_t particle_count;    /* level 8, "Particle count: %d" 8:926 */
+    int32_t border_solid;      /* 1 if a non-fluid cell touched the domain border */
+} ifl_step_stats;
+
+/* ---- lifetime: FluidSolver ctor 3:401 / 4:836 / 5:894 / 6:921 / 7:945 / 8:1289,
+ *      dtor 7:976 ---- */
+typedef struct ifl_solver ifl_solver;
+int ifl_create(const ifl_config* cfg, ifl_solver** out);
+int ifl_destroy(ifl_solver* s);
+const char* ifl_last_error(const ifl_solver* s);   /* s may be NULL */
+const char* ifl_version(void);
+
+/* The borrowed `const vector<const SolidBody*>&` (7:622) that the reference
+ * re-reads on every update(). n <= 255 (uint8 _body, 7:268). Level 8 needs
+ * the bodies before the first update (initParticles 8:735). */
+int ifl_set_bodies(ifl_solver* s, const ifl_body* bodies, int n);
+
+/* FluidSolver::addInflow(x, y, w, h, d, [t,] u, v), 3:449 / 7:1036.
+ * vals = {d, t, u, v}; t is ignored below level 6. */
+int ifl_add_inflow(ifl_solver* s, double x, double y, double w, double h,
+                   const double vals[4]);
+
+/* FluidSolver::update(timestep), 3:433 / 4:869 / 5:927 / 6:966 / 7:995 /
+ * 8:1350. Blocks until the step is done; stats may be NULL. */
+int ifl_update(ifl_solver* s, double dt, ifl_step_stats* stats);
+
+/* ---- state access (the reference's fields are private; the harness and
+ *      teacher-forced tests read and write them through these) ---- */
+enum { IFL_Q_D = 0, IFL_Q_T = 1, IFL_Q_U = 2, IFL_Q_V = 3 };
+enum {
+    IFL_QF_SRC = 0, IFL_QF_DST = 1,     /* _src / _dst (_old at level 8) */
+    IFL_QF_PHI = 2, IFL_QF_VOLUME = 3,
+    IFL_QF_NORMALX = 4, IFL_QF_NORMALY = 5,
+    IFL_QF_CELL = 6, IFL_QF_BODY = 7, IFL_QF_MASK = 8
+};
+#define IFL_FIELD_Q(q, f) ((q) * 16 + (f))
+enum {
+    IFL_FIELD_R = 64, IFL_FIELD_P = 65, IFL_FIELD_Z = 66, IFL_FIELD_S = 67,
+    IFL_FIELD_PRECON = 68, IFL_FIELD_ADIAG = 69, IFL_FIELD_APLUSX = 70,
+    IFL_FIELD_APLUSY = 71, IFL_FIELD_UDENSITY = 72, IFL_FIELD_VDENSITY = 73,
+    /* level 8 (ParticleQuantities 8:692-940) */
+    IFL_FIELD_POSX = 80, IFL_FIELD_POSY = 81,
+    IFL_FIELD_PROP_D = 82, IFL_FIELD_PROP_T = 83, IFL_FIELD_PROP_U = 84,
+    IFL_FIELD_PROP_V = 85, IFL_FIELD_WEIGHT = 86, IFL_FIELD_COUNTS = 87
+};
+
+/* Element count and size (8, 4 or 1 bytes) of a field; 0 count if absent. */
+int ifl_field_info(const ifl_solver* s, int field, size_t* count, int* elem_bytes);
+int ifl_upload(ifl_solver* s, int field, const void* host, size_t bytes);
+int ifl_download(ifl_solver* s, int field, void* host, size_t bytes);
+/* Level 8: the live particle count (_particleCount 8:701). */
+int ifl_get_particle_count(const ifl_solver* s, int* count);
+int ifl_set_particle_count(ifl_solver* s, int count);
+
+/* ---- single stages of update(), for teacher-forced parity tests ----
+ * iarg / darg meaning per stage is given on each line. `result` receives the
+ * scalar of DOT / INF_NORM; `stats` receives PROJECT's solve statistics. */
+enum {
+    IFL_STAGE_FILL_SOLID = 1,       /* iarg=q      FluidQuantity::fillSolidFields 7:475 */
+    IFL_STAGE_EXTRAPOLATE = 2,      /* iarg=q      FluidQuantity::extrapolate 7:560 / 8:611 */
+    IFL_STAGE_ADVECT = 3,           /* iarg=q darg=dt  FluidQuantity::advect 7:437 */
+    IFL_STAGE_FLIP = 4,             /* iarg=q      FluidQuantity::flip 7:354 */
+    IFL_STAGE_BUILD_RHS = 5,        /*             buildRhs 7:624 */
+    IFL_STAGE_BUILD_PRESSURE = 6,   /* darg=dt     buildPressureMatrix 7:680 */
+    IFL_STAGE_BUILD_HEAT = 7,       /* darg=dt     buildHeatDiffusionMatrix 7:709 */
+    IFL_STAGE_BUILD_PRECON = 8,     /*             buildPreconditioner 7:739 */
+    IFL_STAGE_APPLY_PRECON = 9,     /*             applyPreconditioner(_z, _r) 7:770 */
+    IFL_STAGE_MATVEC = 10,          /*             matrixVectorProduct(_z, _s) 7:816 */
+    IFL_STAGE_DOT = 11,             /* iarg=0: dot(_z,_r); 1: dot(_z,_s)  7:806 */
+    IFL_STAGE_INF_NORM = 12,        /*             infinityNorm(_r) 7:843 */
+    IFL_STAGE_PROJECT = 13,         /* iarg=limit  project 7:853 */
+    IFL_STAGE_APPLY_PRESSURE = 14,  /* darg=dt     applyPressure 7:891 */
+    IFL_STAGE_ADD_BUOYANCY = 15,    /* darg=dt     addBuoyancy 7:908 */
+    IFL_STAGE_SET_BOUNDARY = 16,    /*             setBoundaryCondition 7:921 */
+    IFL_STAGE_COMPUTE_DENSITIES = 17, /*           computeDensities 7:658 */
+    IFL_STAGE_COPY_T_TO_R = 18,     /*             memcpy(_r, _t->src()) 7:1001 */
+    IFL_STAGE_COPY_P_TO_T = 19,     /*             memcpy(_t->src(), _p) 7:1005 */
+    /* level 8 */
+    IFL_STAGE_FROM_PARTICLES = 32,  /* iarg=q      fromParticles 8:663 */
+    IFL_STAGE_G2P = 33,             /* darg=alpha  gridToParticles 8:904 */
+    IFL_STAGE_P2G = 34,             /*             particlesToGrid 8:916 */
+    IFL_STAGE_PARTICLE_ADVECT = 35, /* darg=dt     ParticleQuantities::advect 8:931 */
+    IFL_STAGE_COPY_OLD = 36,        /* iarg=q      copy 8:374 */
+    IFL_STAGE_DIFF = 37,            /* iarg=q darg=alpha  diff 8:379 */
+    IFL_STAGE_UNDIFF = 38,          /* iarg=q darg=alpha  undiff 8:385 */
+    IFL_STAGE_COUNT_PARTICLES = 39, /*             countParticles 8:754 */
+    IFL_STAGE_PRUNE = 40,           /*             pruneParticles 8:766 */
+    IFL_STAGE_SEED = 41             /*             seedParticles 8:789 */
+};
+int ifl_stage(ifl_solver* s, int stage, int iarg, double darg,
+              ifl_solve_stats* stats, double* result);
+
+/* ---- timing of the device work, HIP events on the solver's stream ---- */
+enum {
+    IFL_KERNEL_PCG = 0,       /* the persistent project() kernel, one launch per solve */
+    IFL_KERNEL_PRECON = 1,    /* buildPreconditioner wavefront */
+    IFL_KERNEL_ADVECT = 2,
+    IFL_KERNEL_OTHER = 3,
+    IFL_KERNEL_COUNT = 4
+};
+int ifl_profile_enable(ifl_solver* s, int on);
+/* launches, summed milliseconds, summed PCG iterations, summed cell-iterations */
+int ifl_profile_read(const ifl_solver* s, int kernel, int64_t* launches,
+                     double* total_ms, int64_t* iterations, double* cell_iterations);
+int ifl_profile_reset(ifl_solver* s);
+int ifl_synchronize(ifl_solver* s);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* IFL_H */
