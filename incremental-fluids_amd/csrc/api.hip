@@ -1,0 +1,602 @@
+// C ABI of libifl.so (declared in include/ifl.h): solver lifetime, state
+// access, single stages and FluidSolver::update() for solvers 3..8.
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <new>
+
+#include "ifl_solver.hpp"
+
+using namespace ifl;
+
+struct ifl_solver {
+    Solver s;
+};
+
+namespace {
+
+thread_local std::string g_err;
+
+__global__ void fill_u64_kernel(uint64_t* p, long n, uint64_t v) {
+    long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+__global__ void fill_f64_kernel(double* p, long n, double v) {
+    long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+int dalloc(Solver* s, double** p, size_t n) {
+    n = n ? n : 1;
+    IFL_HIP(hipMalloc((void**)p, n * sizeof(double)));
+    IFL_HIP(hipMemsetAsync(*p, 0, n * sizeof(double), s->stream));
+    return IFL_OK;
+}
+int balloc(Solver* s, uint8_t** p, size_t n) {
+    n = (n + 7) & ~(size_t)7;  // whole 32-bit words for the mask atomics
+    IFL_HIP(hipMalloc((void**)p, n));
+    IFL_HIP(hipMemsetAsync(*p, 0, n, s->stream));
+    return IFL_OK;
+}
+int fill(Solver* s, double* p, long n, double v) {
+    fill_f64_kernel<<<grid_blocks(n, 256), 256, 0, s->stream>>>(p, n, v);
+    IFL_LAUNCH_CHECK();
+    return IFL_OK;
+}
+
+bool skewed_field(int f) { return f >= IFL_FIELD_R && f <= IFL_FIELD_APLUSY; }
+
+double* skew_ptr(Solver* s, int f) {
+    switch (f) {
+    case IFL_FIELD_R: return s->r;
+    case IFL_FIELD_P: return s->p;
+    case IFL_FIELD_Z: return s->z;
+    case IFL_FIELD_S: return s->s;
+    case IFL_FIELD_PRECON: return s->precon;
+    case IFL_FIELD_ADIAG: return s->adiag;
+    case IFL_FIELD_APLUSX: return s->aplusx;
+    case IFL_FIELD_APLUSY: return s->aplusy;
+    }
+    return nullptr;
+}
+
+// device pointer, element count and size of a non-skewed field
+void* field_ptr(Solver* s, int f, size_t* count, int* elem) {
+    *count = 0;
+    *elem = 8;
+    int lv = s->level;
+    if (f < 64) {
+        int qi = f / 16, k = f % 16;
+        if (qi > 3 || (qi == IFL_Q_T && lv < 6)) return nullptr;
+        QtyView& q = s->q[qi];
+        size_t n = (size_t)q.w * q.h;
+        *count = n;
+        switch (k) {
+        case IFL_QF_SRC: return q.src;
+        case IFL_QF_DST: return q.dst;
+        case IFL_QF_PHI: if (lv < 5) break; *count = (size_t)(q.w + 1) * (q.h + 1); return q.phi;
+        case IFL_QF_VOLUME: if (lv < 5) break; return q.volume;
+        case IFL_QF_NORMALX: if (lv < 4) break; return q.nx;
+        case IFL_QF_NORMALY: if (lv < 4) break; return q.ny;
+        case IFL_QF_CELL: if (lv < 4) break; *elem = 1; return q.cell;
+        case IFL_QF_BODY: if (lv < 4) break; *elem = 1; return q.body;
+        case IFL_QF_MASK: if (lv < 4) break; *elem = 1; return q.mask;
+        }
+        *count = 0;
+        return nullptr;
+    }
+    size_t n = (size_t)s->W * s->H;
+    switch (f) {
+    case IFL_FIELD_UDENSITY: if (lv < 7) return nullptr; *count = (size_t)(s->W + 1) * s->H; return s->udens;
+    case IFL_FIELD_VDENSITY: if (lv < 7) return nullptr; *count = (size_t)s->W * (s->H + 1); return s->vdens;
+    }
+    if (lv != 8) return nullptr;
+    switch (f) {
+    case IFL_FIELD_POSX: *count = s->max_particles; return s->posx;
+    case IFL_FIELD_POSY: *count = s->max_particles; return s->posy;
+    case IFL_FIELD_PROP_D: case IFL_FIELD_PROP_T: case IFL_FIELD_PROP_U: case IFL_FIELD_PROP_V:
+        *count = s->max_particles;
+        return s->prop[f - IFL_FIELD_PROP_D];
+    case IFL_FIELD_WEIGHT: *count = (size_t)(s->W + 1) * (s->H + 1); return s->weight;
+    case IFL_FIELD_COUNTS: *count = n; *elem = 4; return s->counts;
+    }
+    return nullptr;
+}
+
+void free_all(Solver* s) {
+    auto F = [](void* p) { if (p) (void)hipFree(p); };
+    for (int i = 0; i < 4; i++) {
+        QtyView& q = s->q[i];
+        F(q.src); F(q.dst); F(q.phi); F(q.volume); F(q.nx); F(q.ny); F(q.cell); F(q.body); F(q.mask);
+    }
+    F(s->udens); F(s->vdens);
+    F(s->r); F(s->p); F(s->z); F(s->s); F(s->precon); F(s->adiag); F(s->aplusx); F(s->aplusy);
+    F(s->cx); F(s->cy); F(s->yv); F(s->qv); F(s->dflag); F(s->edge); F(s->partial); F(s->st);
+    F(s->stage_buf); F(s->front); F(s->front_cnt); F(s->bodies_d);
+    F(s->posx); F(s->posy);
+    for (int t = 0; t < 4; t++) F(s->prop[t]);
+    F(s->weight); F(s->counts);
+    if (s->st_host) (void)hipHostFree(s->st_host);
+    if (s->ev0) (void)hipEventDestroy(s->ev0);
+    if (s->ev1) (void)hipEventDestroy(s->ev1);
+    if (s->stream) (void)hipStreamDestroy(s->stream);
+}
+
+int init_solver(Solver* s) {
+    IFL_HIP(hipSetDevice(s->device));
+    IFL_HIP(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking));
+    IFL_HIP(hipEventCreate(&s->ev0));
+    IFL_HIP(hipEventCreate(&s->ev1));
+    int W = s->W, H = s->H, lv = s->level;
+    // FluidSolver ctor 7:953-958: d, t at cell centres, u, v on faces
+    const int qw[4] = {W, W, W + 1, W};
+    const int qh[4] = {H, H, H, H + 1};
+    const double qox[4] = {0.5, 0.5, 0.0, 0.5};
+    const double qoy[4] = {0.5, 0.5, 0.5, 0.0};
+    for (int i = 0; i < 4; i++) {
+        QtyView& q = s->q[i];
+        q.w = qw[i];
+        q.h = qh[i];
+        q.ox = qox[i];
+        q.oy = qoy[i];
+        q.hx = s->hx;
+        size_t n = (size_t)q.w * q.h;
+        IFL_TRY(dalloc(s, &q.src, n));
+        IFL_TRY(dalloc(s, &q.dst, n));
+        IFL_TRY(dalloc(s, &q.phi, (size_t)(q.w + 1) * (q.h + 1)));
+        IFL_TRY(dalloc(s, &q.volume, n));
+        IFL_TRY(dalloc(s, &q.nx, n));
+        IFL_TRY(dalloc(s, &q.ny, n));
+        IFL_TRY(balloc(s, &q.cell, n));
+        IFL_TRY(balloc(s, &q.body, n));
+        IFL_TRY(balloc(s, &q.mask, n));
+        if (lv >= 5) IFL_TRY(fill(s, q.volume, (long)n, 1.0));  // 7:332-335
+    }
+    if (lv >= 6) IFL_TRY(fill(s, s->q[IFL_Q_T].src, (long)W * H, s->cfg.t_amb));  // 7:960-961
+    IFL_TRY(dalloc(s, &s->udens, (size_t)(W + 1) * H));
+    s->vdens_count = (size_t)(W + 1) * (H + 1);
+    IFL_TRY(dalloc(s, &s->vdens, s->vdens_count));
+
+    Skew& sk = s->sk;
+    sk.W = W;
+    sk.H = H;
+    sk.nstrips = (H + kStrip - 1) / kStrip;
+    sk.cols = ((W + kStrip - 1) + 15) / 16 * 16;
+    sk.strip_sz = (long)sk.cols * kStrip;
+    s->sk_count = (size_t)sk.nstrips * sk.strip_sz;
+    size_t n = s->sk_count;
+    double** arrs[] = {&s->r, &s->p, &s->z, &s->s, &s->precon, &s->adiag, &s->aplusx, &s->aplusy,
+                       &s->cx, &s->cy, &s->yv, &s->qv};
+    for (double** a : arrs) IFL_TRY(dalloc(s, a, n));
+    IFL_TRY(balloc(s, &s->dflag, n));
+    long ne = 3L * sk.nstrips * W;
+    IFL_HIP(hipMalloc((void**)&s->edge, ne * sizeof(double)));
+    fill_u64_kernel<<<grid_blocks(ne, 256), 256, 0, s->stream>>>((uint64_t*)s->edge, ne, 0x7FF4DEADBEEF1234ull);
+    IFL_LAUNCH_CHECK();
+    s->npartial = std::max(grid_blocks((long)n, 256), sk.nstrips) + 256;
+    IFL_TRY(dalloc(s, &s->partial, s->npartial));
+    IFL_HIP(hipMalloc((void**)&s->st, sizeof(PcgState)));
+    IFL_HIP(hipMemsetAsync(s->st, 0, sizeof(PcgState), s->stream));
+    IFL_HIP(hipHostMalloc((void**)&s->st_host, sizeof(PcgState), hipHostMallocDefault));
+    memset(s->st_host, 0, sizeof(PcgState));
+    s->stage_count = (size_t)(W + 2) * (H + 2);
+    IFL_TRY(dalloc(s, &s->stage_buf, s->stage_count));
+    long cap = (long)(W + 1) * (H + 1);
+    IFL_HIP(hipMalloc((void**)&s->front, 2 * cap * sizeof(int)));
+    IFL_HIP(hipMalloc((void**)&s->front_cnt, 2 * sizeof(int)));
+    IFL_HIP(hipMemsetAsync(s->front_cnt, 0, 2 * sizeof(int), s->stream));
+    IFL_HIP(hipMalloc((void**)&s->bodies_d, kMaxBodies * sizeof(Body)));
+    IFL_TRY(k_update_dflag(s));
+    IFL_HIP(hipStreamSynchronize(s->stream));
+    return IFL_OK;
+}
+
+int set_err(Solver* s, int rc, const char* msg) {
+    s->err = msg;
+    return rc;
+}
+
+int solve(Solver* s, ifl_solve_stats* st) {
+    ifl_solve_stats local;
+    return k_project(s, s->cfg.pcg_limit, st ? st : &local);
+}
+
+void flip_q(Solver* s, int qi) { std::swap(s->q[qi].src, s->q[qi].dst); }
+
+int read_border_flag(Solver* s) {
+    int h = 0;
+    IFL_HIP(hipMemcpyAsync(&h, s->front_cnt + 1, sizeof(int), hipMemcpyDeviceToHost, s->stream));
+    IFL_HIP(hipStreamSynchronize(s->stream));
+    s->border_solid |= h;
+    return IFL_OK;
+}
+
+int extrapolate(Solver* s, int qi) {
+    IFL_TRY(k_extrapolate(s, qi));
+    return IFL_OK;
+}
+
+int fill_solid(Solver* s, int qi) {
+    IFL_TRY(k_fill_solid(s, qi));
+    if (qi == IFL_Q_D) IFL_TRY(k_update_dflag(s));
+    return IFL_OK;
+}
+
+// FluidSolver::update for levels 3..7 (3:433-447, 4:869-895, 5:927-953,
+// 6:966-1008, 7:995-1034)
+int update_grid(Solver* s, double dt, ifl_step_stats* stats) {
+    int lv = s->level;
+    ifl_solve_stats heat = {IFL_SOLVE_NOT_RUN, 0, 0.0, 0, 0}, press = heat;
+    s->border_solid = 0;
+    if (lv == 3) {
+        IFL_TRY(k_build_rhs(s));
+        IFL_TRY(k_build_pressure(s, dt));
+        IFL_TRY(k_build_precon(s));
+        IFL_TRY(solve(s, &press));
+        IFL_TRY(k_apply_pressure(s, dt));
+        IFL_TRY(k_advect(s, (1 << IFL_Q_D) | (1 << IFL_Q_U) | (1 << IFL_Q_V), dt));
+        flip_q(s, IFL_Q_D);
+        flip_q(s, IFL_Q_U);
+        flip_q(s, IFL_Q_V);
+    } else {
+        IFL_TRY(fill_solid(s, IFL_Q_D));
+        if (lv >= 6) IFL_TRY(fill_solid(s, IFL_Q_T));
+        IFL_TRY(fill_solid(s, IFL_Q_U));
+        IFL_TRY(fill_solid(s, IFL_Q_V));
+        if (lv >= 6) {
+            IFL_TRY(k_copy_t_to_r(s));
+            IFL_TRY(k_build_heat(s, dt));
+            IFL_TRY(k_build_precon(s));
+            IFL_TRY(solve(s, &heat));
+            IFL_TRY(k_copy_p_to_t(s));
+            IFL_TRY(extrapolate(s, IFL_Q_T));
+            IFL_TRY(k_add_buoyancy(s, dt));
+        }
+        IFL_TRY(k_set_boundary(s));
+        IFL_TRY(k_build_rhs(s));
+        if (lv >= 7) IFL_TRY(k_compute_densities(s));
+        IFL_TRY(k_build_pressure(s, dt));
+        IFL_TRY(k_build_precon(s));
+        IFL_TRY(solve(s, &press));
+        IFL_TRY(k_apply_pressure(s, dt));
+        IFL_TRY(extrapolate(s, IFL_Q_D));
+        IFL_TRY(extrapolate(s, IFL_Q_U));
+        IFL_TRY(extrapolate(s, IFL_Q_V));
+        IFL_TRY(read_border_flag(s));
+        IFL_TRY(k_set_boundary(s));
+        int qm = (1 << IFL_Q_D) | (1 << IFL_Q_U) | (1 << IFL_Q_V) | (lv >= 6 ? (1 << IFL_Q_T) : 0);
+        IFL_TRY(k_advect(s, qm, dt));
+        flip_q(s, IFL_Q_D);
+        if (lv >= 6) flip_q(s, IFL_Q_T);
+        flip_q(s, IFL_Q_U);
+        flip_q(s, IFL_Q_V);
+    }
+    IFL_HIP(hipStreamSynchronize(s->stream));
+    if (stats) {
+        stats->heat = heat;
+        stats->pressure = press;
+        stats->particle_count = s->particle_count;
+        stats->border_solid = s->border_solid;
+    }
+    return IFL_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* ifl_version(void) { return "ifl 1.0 (gfx950, solvers 3-8)"; }
+
+const char* ifl_last_error(const ifl_solver* h) {
+    if (!h) return g_err.c_str();
+    return h->s.err.c_str();
+}
+
+int ifl_config_default(ifl_config* cfg, int level, int w, int h) {
+    if (!cfg) return IFL_ERR_ARG;
+    memset(cfg, 0, sizeof *cfg);
+    cfg->level = level;
+    cfg->w = w;
+    cfg->h = h;
+    cfg->precision = IFL_PREC_FP64;
+    cfg->dot_order = IFL_DOT_TREE;
+    cfg->pcg_limit = level == 3 ? 600 : 2000;  // 3:437, 7:1004/1016
+    cfg->verbose = 0;
+    cfg->device = 0;
+    cfg->density = 0.1;          // the reference mains' constants (7:1089-1091)
+    cfg->density_soot = level == 8 ? 0.25 : (level == 7 ? 1.0 : 0.1);
+    cfg->diffusion = 0.01;
+    cfg->tau = 0.97;             // 7:740
+    cfg->sigma = 0.25;           // 7:741
+    cfg->tol = 1e-5;             // 7:859
+    cfg->t_amb = 294.0;          // 7:950
+    cfg->gravity = 9.81;         // 7:951
+    cfg->flip_alpha = 0.001;     // 8:1296
+    cfg->mixed_inner_rtol = 1e-3;
+    cfg->mixed_max_outer = 64;
+    return IFL_OK;
+}
+
+int ifl_create(const ifl_config* cfg, ifl_solver** out) {
+    if (!cfg || !out) return IFL_ERR_ARG;
+    *out = nullptr;
+    if (cfg->level < 3 || cfg->level > 8 || cfg->w < 3 || cfg->h < 3 || (long)cfg->w * cfg->h > (1L << 30)) {
+        g_err = "ifl_create: level must be 3..8 and w, h >= 3";
+        return IFL_ERR_ARG;
+    }
+    if (cfg->level == 8) {
+        g_err = "ifl_create: level 8 (FLIP) is not available in this build";
+        return IFL_ERR_UNSUPPORTED;
+    }
+    if (cfg->precision != IFL_PREC_FP64) {
+        g_err = "ifl_create: only the FP64 form is available in this build";
+        return IFL_ERR_UNSUPPORTED;
+    }
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= cfg->device) {
+        g_err = "ifl_create: no HIP device";
+        return IFL_ERR_NO_DEVICE;
+    }
+    ifl_solver* h = new (std::nothrow) ifl_solver();
+    if (!h) return IFL_ERR_ALLOC;
+    Solver* s = &h->s;
+    s->cfg = *cfg;
+    s->level = cfg->level;
+    s->W = cfg->w;
+    s->H = cfg->h;
+    s->hx = 1.0 / std::min(cfg->w, cfg->h);  // 7:953
+    s->device = cfg->device;
+    int rc = init_solver(s);
+    if (rc != IFL_OK) {
+        g_err = s->err;
+        free_all(s);
+        delete h;
+        return rc;
+    }
+    *out = h;
+    return IFL_OK;
+}
+
+int ifl_destroy(ifl_solver* h) {
+    if (!h) return IFL_OK;
+    (void)hipSetDevice(h->s.device);
+    (void)hipStreamSynchronize(h->s.stream);
+    free_all(&h->s);
+    delete h;
+    return IFL_OK;
+}
+
+int ifl_set_bodies(ifl_solver* h, const ifl_body* b, int n) {
+    if (!h || n < 0 || n > kMaxBodies || (n > 0 && !b)) return IFL_ERR_ARG;
+    Solver* s = &h->s;
+    if (s->level < 4 && n > 0) return set_err(s, IFL_ERR_UNSUPPORTED, "solver 3 has no solid bodies");
+    for (int i = 0; i < n; i++) {
+        Body& d = s->bodies_h[i];
+        d.type = b[i].type;
+        d.px = b[i].pos_x;
+        d.py = b[i].pos_y;
+        d.sx = b[i].scale_x;
+        d.sy = b[i].type == IFL_BODY_SPHERE ? b[i].scale_x : b[i].scale_y;  // SolidSphere 7:223-224
+        d.theta = b[i].theta;
+        d.vx = b[i].vel_x;
+        d.vy = b[i].vel_y;
+        d.vt = b[i].vel_theta;
+        d.cn = b[i].cos_neg;
+        d.sn = b[i].sin_neg;
+        d.cp = b[i].cos_pos;
+        d.sp = b[i].sin_pos;
+    }
+    s->nbodies = n;
+    IFL_HIP(hipSetDevice(s->device));
+    if (n > 0) IFL_HIP(hipMemcpyAsync(s->bodies_d, s->bodies_h, n * sizeof(Body), hipMemcpyHostToDevice, s->stream));
+    IFL_HIP(hipStreamSynchronize(s->stream));
+    return IFL_OK;
+}
+
+int ifl_add_inflow(ifl_solver* h, double x, double y, double w, double hh, const double vals[4]) {
+    if (!h || !vals) return IFL_ERR_ARG;
+    Solver* s = &h->s;
+    IFL_HIP(hipSetDevice(s->device));
+    IFL_TRY(k_add_inflow(s, x, y, x + w, y + hh, vals));  // 7:1036-1041
+    IFL_HIP(hipStreamSynchronize(s->stream));
+    return IFL_OK;
+}
+
+int ifl_update(ifl_solver* h, double dt, ifl_step_stats* stats) {
+    if (!h) return IFL_ERR_ARG;
+    Solver* s = &h->s;
+    IFL_HIP(hipSetDevice(s->device));
+    return update_grid(s, dt, stats);
+}
+
+int ifl_field_info(const ifl_solver* hc, int field, size_t* count, int* elem_bytes) {
+    if (!hc || !count || !elem_bytes) return IFL_ERR_ARG;
+    Solver* s = const_cast<Solver*>(&hc->s);
+    if (skewed_field(field)) {
+        *count = (size_t)s->W * s->H;
+        *elem_bytes = 8;
+        return IFL_OK;
+    }
+    field_ptr(s, field, count, elem_bytes);
+    return *count ? IFL_OK : IFL_ERR_ARG;
+}
+
+int ifl_upload(ifl_solver* h, int field, const void* host, size_t bytes) {
+    if (!h || !host) return IFL_ERR_ARG;
+    Solver* s = &h->s;
+    IFL_HIP(hipSetDevice(s->device));
+    if (skewed_field(field)) {
+        size_t n = (size_t)s->W * s->H;
+        if (bytes != n * 8) return set_err(s, IFL_ERR_ARG, "ifl_upload: size mismatch");
+        IFL_HIP(hipMemcpyAsync(s->stage_buf, host, bytes, hipMemcpyHostToDevice, s->stream));
+        IFL_TRY(k_rows_to_skew(s, s->stage_buf, skew_ptr(s, field)));
+        IFL_HIP(hipStreamSynchronize(s->stream));
+        return IFL_OK;
+    }
+    size_t n;
+    int e;
+    void* p = field_ptr(s, field, &n, &e);
+    if (!p) return set_err(s, IFL_ERR_ARG, "ifl_upload: no such field at this level");
+    if (bytes != n * (size_t)e) return set_err(s, IFL_ERR_ARG, "ifl_upload: size mismatch");
+    IFL_HIP(hipMemcpyAsync(p, host, bytes, hipMemcpyHostToDevice, s->stream));
+    if (field == IFL_FIELD_Q(IFL_Q_D, IFL_QF_CELL)) IFL_TRY(k_update_dflag(s));
+    IFL_HIP(hipStreamSynchronize(s->stream));
+    return IFL_OK;
+}
+
+int ifl_download(ifl_solver* h, int field, void* host, size_t bytes) {
+    if (!h || !host) return IFL_ERR_ARG;
+    Solver* s = &h->s;
+    IFL_HIP(hipSetDevice(s->device));
+    if (skewed_field(field)) {
+        size_t n = (size_t)s->W * s->H;
+        if (bytes != n * 8) return set_err(s, IFL_ERR_ARG, "ifl_download: size mismatch");
+        IFL_TRY(k_skew_to_rows(s, skew_ptr(s, field), s->stage_buf));
+        IFL_HIP(hipMemcpyAsync(host, s->stage_buf, bytes, hipMemcpyDeviceToHost, s->stream));
+        IFL_HIP(hipStreamSynchronize(s->stream));
+        return IFL_OK;
+    }
+    size_t n;
+    int e;
+    void* p = field_ptr(s, field, &n, &e);
+    if (!p) return set_err(s, IFL_ERR_ARG, "ifl_download: no such field at this level");
+    if (bytes != n * (size_t)e) return set_err(s, IFL_ERR_ARG, "ifl_download: size mismatch");
+    IFL_HIP(hipMemcpyAsync(host, p, bytes, hipMemcpyDeviceToHost, s->stream));
+    IFL_HIP(hipStreamSynchronize(s->stream));
+    return IFL_OK;
+}
+
+int ifl_get_particle_count(const ifl_solver* h, int* count) {
+    if (!h || !count) return IFL_ERR_ARG;
+    *count = h->s.particle_count;
+    return h->s.level == 8 ? IFL_OK : IFL_ERR_UNSUPPORTED;
+}
+
+int ifl_set_particle_count(ifl_solver* h, int count) {
+    if (!h) return IFL_ERR_ARG;
+    Solver* s = &h->s;
+    if (s->level != 8) return IFL_ERR_UNSUPPORTED;
+    if (count < 0 || count > s->max_particles) return set_err(s, IFL_ERR_ARG, "particle count out of range");
+    s->particle_count = count;
+    return IFL_OK;
+}
+
+int ifl_stage(ifl_solver* h, int stage, int iarg, double darg, ifl_solve_stats* stats, double* result) {
+    if (!h) return IFL_ERR_ARG;
+    Solver* s = &h->s;
+    int lv = s->level;
+    IFL_HIP(hipSetDevice(s->device));
+    bool qok = iarg >= 0 && iarg < 4 && !(iarg == IFL_Q_T && lv < 6);
+    int rc = IFL_OK;
+    switch (stage) {
+    case IFL_STAGE_FILL_SOLID:
+        if (lv < 4) return IFL_ERR_UNSUPPORTED;
+        if (!qok) return IFL_ERR_ARG;
+        rc = fill_solid(s, iarg);
+        break;
+    case IFL_STAGE_EXTRAPOLATE:
+        if (lv < 4) return IFL_ERR_UNSUPPORTED;
+        if (!qok) return IFL_ERR_ARG;
+        rc = extrapolate(s, iarg);
+        break;
+    case IFL_STAGE_ADVECT:
+        if (lv == 8) return IFL_ERR_UNSUPPORTED;
+        if (!qok) return IFL_ERR_ARG;
+        rc = k_advect(s, 1 << iarg, darg);
+        break;
+    case IFL_STAGE_FLIP:
+        if (lv == 8) return IFL_ERR_UNSUPPORTED;
+        if (!qok) return IFL_ERR_ARG;
+        flip_q(s, iarg);
+        break;
+    case IFL_STAGE_BUILD_RHS: rc = k_build_rhs(s); break;
+    case IFL_STAGE_BUILD_PRESSURE:
+        if (lv >= 7) rc = k_build_pressure(s, darg);
+        else rc = k_build_pressure(s, darg);
+        break;
+    case IFL_STAGE_BUILD_HEAT:
+        if (lv < 6) return IFL_ERR_UNSUPPORTED;
+        rc = k_build_heat(s, darg);
+        break;
+    case IFL_STAGE_BUILD_PRECON: rc = k_build_precon(s); break;
+    case IFL_STAGE_APPLY_PRECON: rc = k_apply_precon(s); break;
+    case IFL_STAGE_MATVEC: rc = k_matvec(s); break;
+    case IFL_STAGE_DOT: {
+        double d = 0;
+        rc = k_dot(s, iarg, &d);
+        if (result) *result = d;
+        break;
+    }
+    case IFL_STAGE_INF_NORM: {
+        double d = 0;
+        rc = k_inf_norm(s, &d);
+        if (result) *result = d;
+        break;
+    }
+    case IFL_STAGE_PROJECT: {
+        ifl_solve_stats st;
+        rc = k_project(s, iarg, &st);
+        if (stats) *stats = st;
+        if (result) *result = st.iterations;
+        break;
+    }
+    case IFL_STAGE_APPLY_PRESSURE: rc = k_apply_pressure(s, darg); break;
+    case IFL_STAGE_ADD_BUOYANCY:
+        if (lv < 6) return IFL_ERR_UNSUPPORTED;
+        rc = k_add_buoyancy(s, darg);
+        break;
+    case IFL_STAGE_SET_BOUNDARY:
+        if (lv < 4) return IFL_ERR_UNSUPPORTED;
+        rc = k_set_boundary(s);
+        break;
+    case IFL_STAGE_COMPUTE_DENSITIES:
+        if (lv < 7) return IFL_ERR_UNSUPPORTED;
+        rc = k_compute_densities(s);
+        break;
+    case IFL_STAGE_COPY_T_TO_R:
+        if (lv < 6) return IFL_ERR_UNSUPPORTED;
+        rc = k_copy_t_to_r(s);
+        break;
+    case IFL_STAGE_COPY_P_TO_T:
+        if (lv < 6) return IFL_ERR_UNSUPPORTED;
+        rc = k_copy_p_to_t(s);
+        break;
+    default:
+        return set_err(s, IFL_ERR_UNSUPPORTED, "ifl_stage: unknown stage or not at this level");
+    }
+    if (rc != IFL_OK) return rc;
+    IFL_HIP(hipStreamSynchronize(s->stream));
+    return IFL_OK;
+}
+
+int ifl_profile_enable(ifl_solver* h, int on) {
+    if (!h) return IFL_ERR_ARG;
+    h->s.profiling = on != 0;
+    return IFL_OK;
+}
+
+int ifl_profile_reset(ifl_solver* h) {
+    if (!h) return IFL_ERR_ARG;
+    for (auto& p : h->s.prof) p = Profile();
+    return IFL_OK;
+}
+
+int ifl_profile_read(const ifl_solver* h, int kernel, int64_t* launches, double* total_ms, int64_t* iterations,
+                     double* cell_iterations) {
+    if (!h || kernel < 0 || kernel >= IFL_KERNEL_COUNT) return IFL_ERR_ARG;
+    const Profile& p = h->s.prof[kernel];
+    if (launches) *launches = p.launches;
+    if (total_ms) *total_ms = p.ms;
+    if (iterations) *iterations = p.iterations;
+    if (cell_iterations) *cell_iterations = p.cell_iterations;
+    return IFL_OK;
+}
+
+int ifl_synchronize(ifl_solver* h) {
+    if (!h) return IFL_ERR_ARG;
+    Solver* s = &h->s;
+    IFL_HIP(hipStreamSynchronize(s->stream));
+    return IFL_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,157 @@
+// Host-side state of one solver instance and the kernel launchers shared
+// between the translation units of libifl.so.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <string>
+
+#include "ifl_common.hpp"
+
+namespace ifl {
+
+constexpr int kMaxBodies = 255;  // _body is uint8_t (7:268)
+
+// Device-side scalars of one project() call (7:853-886). Written and read
+// only by kernels; the host reads `done`, `status`, `iter`, `max_err` once
+// per batch of iterations.
+struct PcgState {
+    double sigma;      // z.r of the current iteration
+    double alpha;
+    double beta;
+    double max_err;
+    double dot_tmp;    // scratch for the sequential-order dot
+    int iter;          // iterations completed
+    int done;          // 1 once converged / early exit / budget
+    int status;        // IFL_SOLVE_*
+    int printed_iter;  // the `iter` the reference prints
+    int timeout;       // a bounded wait gave up (kernel bug, never expected)
+    int pad[3];
+};
+
+struct Profile {
+    long long launches = 0;
+    double ms = 0;
+    long long iterations = 0;
+    double cell_iterations = 0;
+};
+
+struct Solver {
+    ifl_config cfg;
+    int level, W, H;
+    double hx;
+    std::string err;
+    hipStream_t stream = nullptr;
+    int device = 0;
+
+    QtyView q[4];  // d, t, u, v (t only at level >= 6)
+    double* udens = nullptr;  // (w+1)*h, row-major
+    double* vdens = nullptr;  // padded to (w+1)*(h+1): the reference indexes it with the u stride (Q2)
+    size_t vdens_count = 0;
+
+    // PCG arrays, strip-skewed (Skew), each sk.nstrips * sk.strip_sz elements
+    Skew sk;
+    size_t sk_count = 0;
+    double *r = nullptr, *p = nullptr, *z = nullptr, *s = nullptr;
+    double *precon = nullptr, *adiag = nullptr, *aplusx = nullptr, *aplusy = nullptr;
+    double *cx = nullptr, *cy = nullptr;  // aPlusX*precon, aPlusY*precon (the sweeps' products)
+    double *yv = nullptr;                 // forward-sweep output
+    double *qv = nullptr;                 // A*s
+    uint8_t* dflag = nullptr;             // 1 where the d-cell is FLUID (and inside the grid)
+    double* edge = nullptr;               // inter-strip hand-off, 2 buffers of nstrips*W
+    double* partial = nullptr;            // per-block partial reductions
+    int npartial = 0;
+    PcgState* st = nullptr;
+    PcgState* st_host = nullptr;          // pinned mirror
+    double* stage_buf = nullptr;          // row-major staging for layout conversion / upload
+    size_t stage_count = 0;
+
+    // extrapolation frontier lists
+    int* front = nullptr;  // 2 lists of W*H ints + counters
+    int* front_cnt = nullptr;
+
+    // bodies
+    int nbodies = 0;
+    Body bodies_h[kMaxBodies];
+    Body* bodies_d = nullptr;
+
+    // level 8 particles
+    int max_particles = 0;
+    int particle_count = 0;
+    double *posx = nullptr, *posy = nullptr, *prop[4] = {nullptr, nullptr, nullptr, nullptr};
+    double* weight = nullptr;
+    int* counts = nullptr;
+
+    // timing
+    bool profiling = false;
+    Profile prof[IFL_KERNEL_COUNT];
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+
+    int border_solid = 0;
+};
+
+// error helpers
+#define IFL_HIP(call)                                                                    \
+    do {                                                                                 \
+        hipError_t e_ = (call);                                                          \
+        if (e_ != hipSuccess) {                                                          \
+            s->err = std::string(#call) + ": " + hipGetErrorString(e_);                  \
+            return IFL_ERR_HIP;                                                          \
+        }                                                                                \
+    } while (0)
+#define IFL_LAUNCH_CHECK()                                                               \
+    do {                                                                                 \
+        hipError_t e_ = hipGetLastError();                                               \
+        if (e_ != hipSuccess) {                                                          \
+            s->err = std::string("kernel launch: ") + hipGetErrorString(e_);             \
+            return IFL_ERR_HIP;                                                          \
+        }                                                                                \
+    } while (0)
+#define IFL_TRY(call)                  \
+    do {                               \
+        int rc_ = (call);              \
+        if (rc_ != IFL_OK) return rc_; \
+    } while (0)
+
+// ---- launchers (k_fields.hip) ----
+int k_fill_solid(Solver* s, int qi);
+int k_extrapolate(Solver* s, int qi);
+int k_advect(Solver* s, int qmask, double dt);
+int k_add_inflow(Solver* s, double x0, double y0, double x1, double y1, const double vals[4]);
+int k_build_rhs(Solver* s);
+int k_compute_densities(Solver* s);
+int k_build_pressure(Solver* s, double dt);
+int k_build_heat(Solver* s, double dt);
+int k_apply_pressure(Solver* s, double dt);
+int k_add_buoyancy(Solver* s, double dt);
+int k_set_boundary(Solver* s);
+int k_copy_t_to_r(Solver* s);
+int k_copy_p_to_t(Solver* s);
+int k_update_dflag(Solver* s);
+// skewed <-> row-major conversion through s->stage_buf
+int k_skew_to_rows(Solver* s, const double* skew, double* rows);
+int k_rows_to_skew(Solver* s, const double* rows, double* skew);
+
+// ---- launchers (k_pcg.hip) ----
+int k_build_precon(Solver* s);
+int k_apply_precon(Solver* s);       // z = M^-1 r (fluid cells), the stage form
+int k_matvec(Solver* s);             // z = A s, the stage form
+int k_dot(Solver* s, int which, double* out);  // 0: z.r  1: z.s
+int k_inf_norm(Solver* s, double* out);
+int k_project(Solver* s, int limit, ifl_solve_stats* stats);
+
+// ---- launchers (k_flip.hip) ----
+int flip_init_particles(Solver* s);
+int k_from_particles(Solver* s, int qi);
+int k_grid_to_particles(Solver* s, double alpha);
+int k_particles_to_grid(Solver* s);
+int k_particle_advect(Solver* s, double dt);
+int k_copy_old(Solver* s, int qi);
+int k_diff(Solver* s, int qi, double alpha, double sign);
+int k_count_particles(Solver* s);
+int k_prune(Solver* s);
+int k_seed(Solver* s);
+
+inline int grid_blocks(long n, int threads) { return (int)((n + threads - 1) / threads); }
+
+}  // namespace ifl
