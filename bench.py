@@ -1,0 +1,178 @@
+#!/usr/bin/env python3
+"""Benchmark: FluidSolver::update() steps/s of solver 7 (variable density) at
+2048x2048 on one MI355X, plus the PCG's achieved HBM bandwidth.
+
+Workload: the seeded solver-7 scene (ifl_scenes.config3_scene): the
+reference's box, two inflows with seeded temperatures, rho_air 0.1,
+rho_soot 1.0, diffusion 0.01, dt 0.005. Inputs live in HBM before timing.
+`python bench.py --steps K --warmup W` runs W untimed then K timed steps and
+prints one JSON line. The CPU baseline times the reference's own solver 7
+(oracle/_ref) on the host over a bounded sample (see cpu_baseline.sample).
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "incremental-fluids_amd"))
+
+PEAK_HBM_GBS = 8000.0
+# Algorithmic bytes per cell per PCG iteration (matvec 40, axpys + norm 48,
+# forward sweep 40, backward sweep 48, s update 24) plus the 5 one-byte cell
+# flag reads of the FLUID-masked solvers 6-8.
+PCG_BYTES_PER_CELL_ITER = {3: 200.0, 4: 200.0, 5: 200.0, 6: 205.0, 7: 205.0, 8: 205.0}
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--size", type=int, default=2048)
+    ap.add_argument("--form", choices=["fp64", "mixed"], default="fp64")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-iters", type=int, default=30)
+    return ap.parse_args()
+
+
+def cpu_baseline(scene, n, mean_iters):
+    """Times the reference's solver 7 (compiled from its own sources, or the
+    C restatement when that build is absent) on one host core: one update()
+    sequence with both PCG budgets capped, plus a run of PCG iterations, and
+    extrapolates to the GPU run's mean iterations per step."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_lib as ol
+    kind = "reference" if ol.ref_available(7) else "port"
+    S = ol.RefSolver if kind == "reference" else ol.OracleSolver
+    s = S(7, n, n, scene["params"], scene["bodies"])
+    for inf in scene["inflows"]:
+        s.add_inflow(*inf)
+    dt = scene["dt"]
+    cap = 2
+    seq = [("FILL_SOLID", 0), ("FILL_SOLID", 1), ("FILL_SOLID", 2), ("FILL_SOLID", 3), ("COPY_T_TO_R", 0),
+           ("BUILD_HEAT", 0), ("BUILD_PRECON", 0), ("PROJECT", cap), ("COPY_P_TO_T", 0), ("EXTRAPOLATE", 1),
+           ("ADD_BUOYANCY", 0), ("SET_BOUNDARY", 0), ("BUILD_RHS", 0), ("COMPUTE_DENSITIES", 0),
+           ("BUILD_PRESSURE", 0), ("BUILD_PRECON", 0), ("PROJECT", cap), ("APPLY_PRESSURE", 0),
+           ("EXTRAPOLATE", 0), ("EXTRAPOLATE", 2), ("EXTRAPOLATE", 3), ("SET_BOUNDARY", 0),
+           ("ADVECT", 0), ("ADVECT", 1), ("ADVECT", 2), ("ADVECT", 3)]
+    t0 = time.perf_counter()
+    for name, ia in seq:
+        s.stage(name, ia, dt)
+    t_seq = time.perf_counter() - t0
+    # PCG iterations: rebuild the pressure system and run a capped solve
+    s.stage("BUILD_RHS")
+    s.stage("BUILD_PRESSURE", 0, dt)
+    s.stage("BUILD_PRECON")
+    t0 = time.perf_counter()
+    s.stage("PROJECT", 0)
+    t_p0 = time.perf_counter() - t0
+    s.stage("BUILD_RHS")
+    t0 = time.perf_counter()
+    s.stage("PROJECT", cpu_iters_global)
+    t_pk = time.perf_counter() - t0
+    t_iter = max(t_pk - t_p0, 0.0) / cpu_iters_global
+    t_fixed = t_seq - 2 * cap * t_iter
+    step_s = t_fixed + mean_iters * t_iter
+    if hasattr(s, "close"):
+        s.close()
+    return dict(value=1.0 / step_s, unit="steps/s", cores=1, kind=kind,
+                sample=(f"solver 7 at {n}x{n} on one host core: one update() sequence with both PCG budgets "
+                        f"capped at {cap} ({t_seq:.2f} s) and {cpu_iters_global} PCG iterations "
+                        f"({t_iter * 1e3:.1f} ms each); step time extrapolated to the GPU run's mean "
+                        f"{mean_iters:.0f} PCG iterations/step = {step_s:.1f} s"),
+                seconds_per_step=step_s, ms_per_pcg_iteration=t_iter * 1e3)
+
+
+cpu_iters_global = 12
+
+
+def main():
+    global cpu_iters_global
+    args = parse()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 "
+                                   "(the multi-GPU strip decomposition is a design note only)"}))
+        sys.exit(2)
+    cpu_iters_global = args.cpu_iters
+    import ifl_fluids as F
+    import ifl_scenes as S
+
+    n = args.size
+    scene = S.config3_scene()
+    prec = F.PREC_MIXED if args.form == "mixed" else F.PREC_FP64
+    objs = [F.SolidBox(*b[1:]) if b[0] == S.BOX else F.SolidSphere(b[1], b[2], b[3], *b[5:]) for b in scene["bodies"]]
+    solver = F.FluidSolver(7, n, n, scene["params"], objs, precision=prec)
+
+    iters = []
+
+    def on_step(k, st):
+        iters.append((st.heat.iterations, st.pressure.iterations, st.heat.status, st.pressure.status))
+
+    S.drive(solver, scene, args.warmup)
+    solver.synchronize()
+    iters.clear()
+    solver.profile(True)
+    solver.profile_reset()
+    solver.synchronize()
+    t0 = time.perf_counter()
+    steps_done = [0]
+
+    def timed_step(k, st):
+        on_step(k, st)
+        steps_done[0] += 1
+
+    # the scene's body motion cadence continues from the warmup
+    for k in range(args.steps):
+        for inf in scene["inflows"]:
+            solver.add_inflow(*inf)
+        st = solver.update(scene["dt"])
+        timed_step(k, st)
+        if (args.warmup + k + 1) % scene["substeps"] == 0:
+            solver.bodies_update(scene["dt"])
+    solver.synchronize()
+    wall = time.perf_counter() - t0
+    prof = solver.profile_read(F.KERNEL_PCG)
+
+    steps_per_s = args.steps / wall
+    pcg_iters = prof["iterations"]
+    bpc = PCG_BYTES_PER_CELL_ITER[7]
+    pcg_gbs = bpc * prof["cell_iterations"] / (prof["ms"] * 1e-3) / 1e9 if prof["ms"] > 0 else 0.0
+    mean_iters = pcg_iters / max(1, args.steps)
+
+    out = {
+        "metric": "solver steps/s at 2048^2 (solver 7, FP64 drop-in); PCG HBM GB/s",
+        "value": steps_per_s,
+        "unit": "steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": 1e3 * wall / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64" if args.form == "fp64" else "f32+f64",
+        "data": "synthetic",
+        "config": {"workload": f"solver7-variable-density-{n}x{n}-box-2inflows-seeded",
+                   "grid": [n, n], "level": 7, "form": args.form, "dt": scene["dt"],
+                   "pcg_iterations_per_step": mean_iters,
+                   "solves": [list(x) for x in iters]},
+        "pcg": {"launches": prof["launches"], "ms_total": prof["ms"], "iterations": pcg_iters,
+                "ms_per_iteration": prof["ms"] / max(1, pcg_iters), "bytes_per_cell_iteration": bpc,
+                "achieved_GBps": pcg_gbs},
+        "roofline": {"bound": "hbm", "achieved": pcg_gbs, "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                     "frac": pcg_gbs / PEAK_HBM_GBS, "traffic": None,
+                     "kernel": "project() (one MIC(0)-PCG solve), algorithmic bytes 205 B/cell/iteration"},
+    }
+    solver.close()
+    if not args.no_cpu_baseline:
+        try:
+            out["cpu_baseline"] = cpu_baseline(scene, n, mean_iters)
+        except Exception as e:  # the baseline is reported, never the measured value
+            out["cpu_baseline"] = {"error": str(e)}
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

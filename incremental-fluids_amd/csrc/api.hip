@@ -127,6 +127,9 @@ int init_solver(Solver* s) {
     IFL_HIP(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking));
     IFL_HIP(hipEventCreate(&s->ev0));
     IFL_HIP(hipEventCreate(&s->ev1));
+    int lds = 0;
+    IFL_HIP(hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, s->device));
+    s->lds_bytes = (size_t)lds;
     int W = s->W, H = s->H, lv = s->level;
     // FluidSolver ctor 7:953-958: d, t at cell centres, u, v on faces
     const int qw[4] = {W, W, W + 1, W};

@@ -4,6 +4,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <cstring>
 #include <string>
 
 #include "ifl_common.hpp"
@@ -88,6 +89,7 @@ struct Solver {
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
 
     int border_solid = 0;
+    size_t lds_bytes = 65536;  // dynamic LDS available to one workgroup
 };
 
 // error helpers

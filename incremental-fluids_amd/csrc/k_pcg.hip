@@ -75,253 +75,365 @@ IFL_DEV double lane_value(double v, int lane) {
     return __hiloint2double(hi, lo);
 }
 
-struct SweepArgs {
-    Skew sk;
-    double* r;
-    const double* q;
-    double* y;
-    double* z;
-    const double* cx;
-    const double* cy;
-    const double* pc;
-    const uint8_t* flag;
-    double* edge;     // nstrips * W
-    double* partial;  // per strip
-    PcgState* st;
-    int mode;         // runtime copy of MODE for logging
+
+// ---------------------------------------------------------------- streamed sweeps
+// A sweep step needs ~2.5 KB of fresh data per strip while one wave can keep
+// only a few hundred nanoseconds of loads in flight, far less than the HBM
+// latency. Each strip therefore runs as a workgroup of
+//   wave 0        the recurrence, reading its operands from an LDS ring;
+//   wave 1        the edge poller, fetching 64 upcoming inter-strip values
+//                 per round trip into a small LDS ring;
+//   waves 2..6    loaders, each streaming every kLoaders-th block of 8 steps
+//                 (4 KB contiguous per array in the skewed layout) from HBM
+//                 into the LDS ring, with kDepth blocks in flight.
+// Ring slots are recycled through counters in LDS (acquire/release at
+// workgroup scope). Every wait is bounded; a timeout raises `abort`, which
+// every wave checks, so the workgroup always drains.
+constexpr int kBlk = 8;
+constexpr int kLoaders = 5;
+constexpr int kWaves = 2 + kLoaders;
+constexpr int kDepth = 2;
+constexpr int kERing = 512;  // edge ring, in steps
+constexpr int kMaxArrays = 6;
+constexpr long kLdsSpin = 1L << 22;
+
+enum { SW_FWD = 0, SW_BWD = 1, SW_PRECON = 2 };
+
+struct RingHdr {
+    int consumed;            // blocks the compute wave has finished with
+    int ready[kLoaders];     // per loader: 1 + last block published
+    int edge_ready;          // steps whose edge value is in the ring
+    int abort;
+    int pad[8];
 };
 
-// ---------------------------------------------------------------- forward
-// applyPreconditioner forward loop 7:773-787, fused (MODE 1) with
-// scaledAdd(_r, _r, _z, -alpha) 7:870 and infinityNorm(_r) 7:872.
-// PM: precondition only FLUID cells (levels 4-8). VM: vector ops only on
-// FLUID cells (levels 6-8).
-template <bool PM, bool VM, int MODE>
-__global__ __launch_bounds__(64) void fwd_sweep_kernel(SweepArgs a) {
-    const int j = strip_of_block(blockIdx.x, a.sk.nstrips);
-    if (j < 0) return;
-    PcgState* st = a.st;
-    if ((MODE == 1 && st->done) || st->timeout) return;
-    const int l = threadIdx.x;
-    const int W = a.sk.W, H = a.sk.H, cols = a.sk.cols;
-    const bool row_ok = j * kStrip + l < H;
-    const long base = (long)j * a.sk.strip_sz + l;
-    const double alpha_neg = MODE == 1 ? -st->alpha : 0.0;
-    double* ein = j > 0 ? a.edge + (long)j * W : nullptr;
-    double* eout = j + 1 < a.sk.nstrips ? a.edge + (long)(j + 1) * W : nullptr;
-    double Pw = 0.0, Pn = 0.0, rmax = 0.0;
-
-    double nr[kU], nq[kU], ncx[kU], ncy[kU], npc[kU];
-    uint8_t nf[kU];
-    auto load = [&](int T) {
-#pragma unroll
-        for (int k = 0; k < kU; k++) {
-            long p = base + (long)(T + k) * kStrip;
-            nr[k] = a.r[p];
-            if (MODE == 1) nq[k] = a.q[p];
-            ncx[k] = a.cx[p];
-            ncy[k] = a.cy[p];
-            npc[k] = a.pc[p];
-            if (PM || VM) nf[k] = a.flag[p];
-        }
-    };
-    load(0);
-    for (int T = 0; T < cols; T += kU) {
-        double cr[kU], cq[kU], ccx[kU], ccy[kU], cpc[kU];
-        uint8_t cf[kU];
-#pragma unroll
-        for (int k = 0; k < kU; k++) {
-            cr[k] = nr[k]; cq[k] = nq[k]; ccx[k] = ncx[k]; ccy[k] = ncy[k]; cpc[k] = npc[k];
-            cf[k] = (PM || VM) ? nf[k] : 1;
-        }
-        if (T + kU < cols) load(T + kU);
-        double eblk = 0.0;
-        if (ein) eblk = edge_fetch(ein, T + l, l < kU && T + l < W, &st->timeout);
-#pragma unroll
-        for (int k = 0; k < kU; k++) {
-            const int t = T + k;
-            const int x = t - l;
-            const bool valid = row_ok && x >= 0 && x < W;
-            const double ev = ein ? lane_value(eblk, k) : 0.0;
-            const double Ps = wave_shr1(Pn, ev);
-            double r = cr[k];
-            const bool vec_on = valid && (!VM || cf[k]);
-            if (MODE == 1) {
-                double rn = r + cq[k] * alpha_neg;
-                if (vec_on) {
-                    r = rn;
-                    a.r[base + (long)t * kStrip] = rn;
-                }
-            }
-            if (vec_on) rmax = smax(rmax, fabs(r));
-            const double t1 = r - Pw;
-            const double tt = t1 - Ps;
-            const double zz = tt * cpc[k];
-            double pw = ccx[k] * zz;
-            double pn = ccy[k] * zz;
-            if (PM) {
-                pw = cf[k] ? pw : 0.0;
-                pn = cf[k] ? pn : 0.0;
-            }
-            Pw = pw;
-            Pn = pn;
-            if (valid && (!PM || cf[k])) a.y[base + (long)t * kStrip] = zz;
-            if (eout && l == kStrip - 1 && valid) st_relaxed(eout + x, as_bits(pn));
-        }
-    }
-    rmax = wave_max(rmax);
-    if (l == 0) atomicMax((unsigned long long*)&st->max_err, (unsigned long long)as_bits(rmax));
-}
-
-// ---------------------------------------------------------------- backward
-// applyPreconditioner backward loop 7:789-803 fused with dotProduct(_z, _r)
-// 7:880. Lane l runs x = W-1 - (t - (63-l)), so lane 63 leads. MODE 1 also
-// stores z = A s (held in q) at non-FLUID cells, which is what the
-// reference's _z holds there after matrixVectorProduct.
-template <bool PM, bool VM, int MODE>
-__global__ __launch_bounds__(64) void bwd_sweep_kernel(SweepArgs a) {
-    const int j = strip_of_block(blockIdx.x, a.sk.nstrips);
-    if (j < 0) return;
-    PcgState* st = a.st;
-    if ((MODE == 1 && st->done) || st->timeout) return;
-    const int l = threadIdx.x;
-    const int W = a.sk.W, H = a.sk.H, cols = a.sk.cols;
-    const bool row_ok = j * kStrip + l < H;
-    const long base = (long)j * a.sk.strip_sz + l;
-    double* ein = j + 1 < a.sk.nstrips ? a.edge + (long)(j + 1) * W : nullptr;  // from the strip above
-    double* eout = j > 0 ? a.edge + (long)j * W : nullptr;                     // to the strip below
-    double zs = 0.0, acc = 0.0;
-
-    double ny_[kU], ncx[kU], ncy[kU], npc[kU], nr[kU];
-    uint8_t nf[kU];
-    auto load = [&](int T) {
-#pragma unroll
-        for (int k = 0; k < kU; k++) {
-            long c = cols - 1 - (T + k);
-            long p = base + c * kStrip;
-            ny_[k] = a.y[p];
-            ncx[k] = a.cx[p];
-            ncy[k] = a.cy[p];
-            npc[k] = a.pc[p];
-            nr[k] = a.r[p];
-            if (PM || VM) nf[k] = a.flag[p];
-        }
-    };
-    load(0);
-    for (int T = 0; T < cols; T += kU) {
-        double cy_[kU], ccx[kU], ccy[kU], cpc[kU], cr[kU];
-        uint8_t cf[kU];
-#pragma unroll
-        for (int k = 0; k < kU; k++) {
-            cy_[k] = ny_[k]; ccx[k] = ncx[k]; ccy[k] = ncy[k]; cpc[k] = npc[k]; cr[k] = nr[k];
-            cf[k] = (PM || VM) ? nf[k] : 1;
-        }
-        if (T + kU < cols) load(T + kU);
-        // lane 63 of this strip consumes x = cols-1-(T+k)-63 at step k
-        double eblk = 0.0;
-        if (ein) {
-            int xe = cols - 1 - (T + l) - (kStrip - 1);
-            eblk = edge_fetch(ein, xe, l < kU && xe >= 0 && xe < W, &st->timeout);
-        }
-#pragma unroll
-        for (int k = 0; k < kU; k++) {
-            const long c = cols - 1 - (T + k);
-            const int x = (int)c - l;
-            const bool valid = row_ok && x >= 0 && x < W;
-            const long p = base + c * kStrip;
-            const double ev = ein ? lane_value(eblk, k) : 0.0;
-            const double zn = wave_shl1(zs, ev);
-            const double t1 = cy_[k] - ccx[k] * zs;
-            const double tt = t1 - ccy[k] * zn;
-            const double zz = tt * cpc[k];
-            const bool fl = cf[k] != 0;
-            zs = PM ? (fl ? zz : 0.0) : zz;
-            double zdot = zz;
-            if (valid) {
-                if (!PM || fl) {
-                    a.z[p] = zz;
-                } else if (MODE == 1) {
-                    zdot = a.q[p];
-                    a.z[p] = zdot;
-                } else {
-                    zdot = a.z[p];  // the stale value the reference's _z holds there
-                }
-                if (!VM || fl) acc += zdot * cr[k];
-            }
-            if (eout && l == 0 && valid) st_relaxed(eout + x, as_bits(zs));
-        }
-    }
-    acc = wave_sum(acc);
-    if (l == 0) a.partial[j] = acc;
-}
-
-// ---------------------------------------------------------------- precon
-// buildPreconditioner 7:739-768 (3:247-272 with every cell FLUID). A FLUID
-// cell passes (px*px + tau*px*py) east and (py*py + tau*px*py) north, with
-// px = aPlusX*precon and py = aPlusY*precon of its own; the receiver
-// subtracts them in the reference's order. px and py are kept as the
-// sweeps' coefficients (aPlusX[i]*precon[i] is the product the sweeps use).
-struct PreconArgs {
+struct StreamArgs {
     Skew sk;
-    const double *adiag, *apx, *apy;
-    double *pc, *cx, *cy;
+    const double* in[kMaxArrays];
+    int na;
     const uint8_t* flag;
     double* edge;
+    double* out0;  // fwd: y   bwd: z   precon: precon
+    double* out1;  // fwd: r   precon: cx
+    double* out2;  // precon: cy
+    double* partial;
+    PcgState* st;
     int* timeout;
     double tau, sigma;
+    int nb;          // ring depth in blocks
+    int slot_bytes;  // na*4096 + 512
 };
 
-__global__ __launch_bounds__(64) void precon_kernel(PreconArgs a) {
+IFL_DEV int lds_acq(int* p) { return __hip_atomic_load(p, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_WORKGROUP); }
+IFL_DEV void lds_rel(int* p, int v) { __hip_atomic_store(p, v, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_WORKGROUP); }
+
+// Spin until *p > bound (or abort); false when aborted.
+IFL_DEV bool lds_wait_gt(int* p, int bound, RingHdr* hdr, int* gtimeout) {
+    long spins = 0;
+    while (lds_acq(p) <= bound) {
+        if (lds_acq(&hdr->abort)) return false;
+        if (++spins > kLdsSpin) {
+            lds_rel(&hdr->abort, 1);
+            atomicOr(gtimeout, 1);
+            return false;
+        }
+        __builtin_amdgcn_s_sleep(1);
+    }
+    return true;
+}
+
+// first skewed column of block b (8 columns, ascending in memory)
+template <int KIND>
+IFL_DEV int block_col0(int b, int cols) {
+    return KIND == SW_BWD ? cols - kBlk - b * kBlk : b * kBlk;
+}
+// the column lane-0 (fwd/precon) or lane-63 (bwd) edge value of step t belongs to
+template <int KIND>
+IFL_DEV int edge_x(int t, int cols) {
+    return KIND == SW_BWD ? cols - kStrip - t : t;
+}
+
+// s_waitcnt with only vmcnt constrained (gfx9 encoding: vmcnt in bits 3:0
+// and 15:14, expcnt 6:4 and lgkmcnt 11:8 left at their maxima).
+#define IFL_WAIT_VMCNT(n) __builtin_amdgcn_s_waitcnt(((n) & 0xF) | (0x7 << 4) | (0xF << 8) | (((n) >> 4) << 14))
+
+typedef __attribute__((address_space(3))) char lds_char;
+
+// LDS counter access for the loader waves, written as inline asm: the
+// compiler would otherwise wait for every in-flight LDS-DMA load (vmcnt(0))
+// before any LDS instruction, serialising the load pipeline.
+IFL_DEV unsigned lds_off(const void* p) { return (unsigned)(size_t)(const __attribute__((address_space(3))) char*)p; }
+IFL_DEV int lds_ld_raw(const int* p) {
+    int v;
+    asm volatile("ds_read_b32 %0, %1\n\ts_waitcnt lgkmcnt(0)" : "=v"(v) : "v"(lds_off(p)) : "memory");
+    return v;
+}
+IFL_DEV void lds_st_raw(int* p, int v) {
+    asm volatile("ds_write_b32 %0, %1\n\ts_waitcnt lgkmcnt(0)" : : "v"(lds_off(p)), "v"(v) : "memory");
+}
+IFL_DEV bool lds_wait_gt_raw(const int* p, int bound, RingHdr* hdr, int* gtimeout) {
+    long spins = 0;
+    while (lds_ld_raw(p) <= bound) {
+        if (lds_ld_raw(&hdr->abort)) return false;
+        if (++spins > kLdsSpin) {
+            lds_st_raw(&hdr->abort, 1);
+            atomicOr(gtimeout, 1);
+            return false;
+        }
+        __builtin_amdgcn_s_sleep(1);
+    }
+    return true;
+}
+
+template <int KIND, int NA>
+IFL_DEV void loader_wave(const StreamArgs& a, char* ring, RingHdr* hdr, int k, int lane, long sbase, int nblk) {
+    const int cols = a.sk.cols;
+    constexpr int kLoadsPerBlock = NA * 4 + 2;
+    // Loads land in LDS directly (global_load_lds_dwordx4: lane i fills bytes
+    // [16 i, 16 i + 16) of the 1 KB run at the wave-uniform LDS address).
+    auto issue = [&](int b) {
+        long off = sbase + (long)block_col0<KIND>(b, cols) * kStrip;
+        lds_char* slot = (lds_char*)(ring + (long)(b % a.nb) * a.slot_bytes);
+#pragma unroll
+        for (int ai = 0; ai < NA; ai++)
+#pragma unroll
+            for (int m = 0; m < 4; m++)
+                __builtin_amdgcn_global_load_lds((const void*)(a.in[ai] + off + m * 128 + lane * 2),
+                                                 (__attribute__((address_space(3))) void*)(slot + ai * 4096 + m * 1024), 16, 0, 0);
+#pragma unroll
+        for (int m = 0; m < 2; m++)
+            __builtin_amdgcn_global_load_lds((const void*)(a.flag + off + m * 256 + lane * 4),
+                                             (__attribute__((address_space(3))) void*)(slot + NA * 4096 + m * 256), 4, 0, 0);
+    };
+    const int nmine = nblk > k ? (nblk - k + kLoaders - 1) / kLoaders : 0;
+    if (nmine == 0) return;
+    if (!lds_wait_gt_raw(&hdr->consumed, k - a.nb, hdr, a.timeout)) return;
+    issue(k);
+    for (int i = 0; i < nmine; i++) {
+        const int b = k + i * kLoaders;
+        const int bn = b + kLoaders;
+        const bool has_next = i + 1 < nmine;
+        // Put the next block in flight only if its slot is already free:
+        // waiting for it here could wait on the compute wave, which may be
+        // waiting for block b.
+        bool issued = false;
+        if (has_next && lds_ld_raw(&hdr->consumed) > bn - a.nb) {
+            issue(bn);
+            issued = true;
+            IFL_WAIT_VMCNT(kLoadsPerBlock);  // block b has landed, bn may still be in flight
+        } else {
+            IFL_WAIT_VMCNT(0);
+        }
+        lds_st_raw(&hdr->ready[k], b + 1);
+        if (has_next && !issued) {
+            if (!lds_wait_gt_raw(&hdr->consumed, bn - a.nb, hdr, a.timeout)) return;
+            issue(bn);
+        }
+    }
+    IFL_WAIT_VMCNT(0);
+}
+
+template <int KIND>
+IFL_DEV void poller_wave(const StreamArgs& a, double* ering, RingHdr* hdr, int lane) {
+    const int cols = a.sk.cols, W = a.sk.W;
+    int t0 = 0;
+    long spins = 0;
+    while (t0 < cols) {
+        if (lds_acq(&hdr->abort)) return;
+        // ring capacity: steps below consumed*kBlk are free
+        int cap = lds_acq(&hdr->consumed) * kBlk + kERing;
+        int t = t0 + lane;
+        int x = edge_x<KIND>(t, cols);
+        bool inwin = t < cols && t < cap;
+        bool need = inwin && x >= 0 && x < W;
+        uint64_t v = need ? ld_relaxed(a.edge + x) : 0;
+        bool ok = inwin && (!need || v != kSentinel);
+        uint64_t m = __ballot(ok);
+        int n = m == ~0ull ? 64 : (int)__builtin_ctzll(~m);  // ready prefix length
+        if (lane < n) {
+            ering[t % kERing] = as_double(v);
+            if (need) st_relaxed(a.edge + x, kSentinel);
+        }
+        if (n > 0) {
+            t0 += n;
+            lds_rel(&hdr->edge_ready, t0);
+            spins = 0;
+        } else {
+            if (++spins > kLdsSpin) {
+                lds_rel(&hdr->abort, 1);
+                atomicOr(a.timeout, 1);
+                return;
+            }
+            __builtin_amdgcn_s_sleep(2);
+        }
+    }
+}
+
+template <int KIND, bool PM, bool VM, int MODE, int NA>
+__global__ __launch_bounds__(64 * kWaves) void stream_sweep_kernel(StreamArgs a) {
+    extern __shared__ __align__(16) char smem[];
     const int j = strip_of_block(blockIdx.x, a.sk.nstrips);
     if (j < 0) return;
-    const int l = threadIdx.x;
+    PcgState* st = a.st;
+    if (KIND != SW_PRECON && ((MODE == 1 && st->done) || st->timeout)) return;
+    RingHdr* hdr = (RingHdr*)smem;
+    double* ering = (double*)(smem + sizeof(RingHdr));
+    char* ring = smem + sizeof(RingHdr) + kERing * sizeof(double);
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
+    if (threadIdx.x < (int)(sizeof(RingHdr) / sizeof(int))) ((int*)hdr)[threadIdx.x] = 0;
+    __syncthreads();
+
     const int W = a.sk.W, H = a.sk.H, cols = a.sk.cols;
-    const bool row_ok = j * kStrip + l < H;
-    const long base = (long)j * a.sk.strip_sz + l;
-    double* ein = j > 0 ? a.edge + (long)j * W : nullptr;
-    double* eout = j + 1 < a.sk.nstrips ? a.edge + (long)(j + 1) * W : nullptr;
+    const int nblk = cols / kBlk;
+    const long sbase = (long)j * a.sk.strip_sz;
+    // edge source: the strip below (fwd, precon) or above (bwd)
+    const bool has_in = KIND == SW_BWD ? j + 1 < a.sk.nstrips : j > 0;
+    const bool has_out = KIND == SW_BWD ? j > 0 : j + 1 < a.sk.nstrips;
+    double* ein = has_in ? a.edge + (long)(KIND == SW_BWD ? j + 1 : j) * W : nullptr;
+    double* eout = has_out ? a.edge + (long)(KIND == SW_BWD ? j : j + 1) * W : nullptr;
+
+    if (wave >= 2) {
+        loader_wave<KIND, NA>(a, ring, hdr, wave - 2, lane, sbase, nblk);
+        return;
+    }
+    if (wave == 1) {
+        if (ein) {
+            StreamArgs b = a;
+            b.edge = ein;
+            poller_wave<KIND>(b, ering, hdr, lane);
+        }
+        return;
+    }
+
+    // ---- wave 0: the recurrence ----
+    const bool row_ok = j * kStrip + lane < H;
+    const long gbase = sbase + lane;
+    constexpr int na = NA;
+    const double alpha_neg = (KIND == SW_FWD && MODE == 1) ? -st->alpha : 0.0;
     const double tau = a.tau, sigma = a.sigma;
-    double Ew = 0.0, En = 0.0;
-    for (int T = 0; T < cols; T += kU) {
-        double ad[kU], px_[kU], py_[kU];
-        uint8_t f[kU];
+    // 4-step rolling window of operands read from the ring (3 steps ahead)
+    double w0[4], w1[4], w2[4], w3[4], w4[4], w5[4], wev[4];
+    uint8_t wf[4];
+    auto rd = [&](int t, int wi) {
+        const int b = t >> 3, k = t & 7;
+        const char* slot = ring + (long)(b % a.nb) * a.slot_bytes;
+        const double* s0 = (const double*)slot;
+        const int e = (KIND == SW_BWD ? (kBlk - 1 - k) : k) * kStrip + lane;
+        w0[wi] = s0[e];
+        w1[wi] = s0[512 + e];
+        w2[wi] = s0[1024 + e];
+        if (na > 3) w3[wi] = s0[1536 + e];
+        if (na > 4) w4[wi] = s0[2048 + e];
+        if (na > 5) w5[wi] = s0[2560 + e];
+        wf[wi] = ((const uint8_t*)(slot + na * 4096))[e];
+        wev[wi] = ein ? ering[t % kERing] : 0.0;
+    };
+    auto ready = [&](int b) -> bool {
+        if (!lds_wait_gt(&hdr->ready[b % kLoaders], b, hdr, a.timeout)) return false;
+        if (ein && !lds_wait_gt(&hdr->edge_ready, b * kBlk + kBlk - 1, hdr, a.timeout)) return false;
+        return true;
+    };
+
+    double Pw = 0.0, Pn = 0.0, rmax = 0.0;  // fwd / precon carries (precon: Ew, En)
+    double zs = 0.0, acc = 0.0;             // bwd carries
+    if (!ready(0)) return;
+    rd(0, 0);
+    rd(1, 1);
+    rd(2, 2);
+    for (int b = 0; b < nblk; b++) {
+        if (b + 1 < nblk && !ready(b + 1)) return;
+        const int T = b * kBlk;
 #pragma unroll
-        for (int k = 0; k < kU; k++) {
-            long p = base + (long)(T + k) * kStrip;
-            ad[k] = a.adiag[p];
-            px_[k] = a.apx[p];
-            py_[k] = a.apy[p];
-            f[k] = a.flag[p];
-        }
-        double eblk = 0.0;
-        if (ein) eblk = edge_fetch(ein, T + l, l < kU && T + l < W, a.timeout);
-#pragma unroll
-        for (int k = 0; k < kU; k++) {
+        for (int k = 0; k < kBlk; k++) {
             const int t = T + k;
-            const int x = t - l;
-            const bool valid = row_ok && x >= 0 && x < W;
-            const double ev = ein ? lane_value(eblk, k) : 0.0;
-            const double Ns = wave_shr1(En, ev);
-            double e = ad[k];
-            e = e - Ew;
-            e = e - Ns;
-            if (e < sigma * ad[k]) e = ad[k];
-            const double pcv = 1.0 / sqrt(e);
-            const double px = px_[k] * pcv;
-            const double py = py_[k] * pcv;
-            const double eo = px * px + tau * px * py;
-            const double no = py * py + tau * px * py;
-            const bool on = valid && f[k];
-            Ew = on ? eo : 0.0;
-            En = on ? no : 0.0;
-            if (on) {
-                long p = base + (long)t * kStrip;
-                a.pc[p] = pcv;
-                a.cx[p] = px;
-                a.cy[p] = py;
+            if (t + 3 < cols) rd(t + 3, (k + 3) & 3);
+            const int wi = k & 3;
+            const bool fl = wf[wi] != 0;
+            if (KIND == SW_FWD) {
+                // applyPreconditioner forward loop 7:773-787 (+ scaledAdd / infinityNorm in MODE 1)
+                const int x = t - lane;
+                const bool valid = row_ok && x >= 0 && x < W;
+                const long p = gbase + (long)t * kStrip;
+                const double Ps = wave_shr1(Pn, wev[wi]);
+                double r = w0[wi];
+                const bool vec_on = valid && (!VM || fl);
+                if (MODE == 1) {
+                    double rn = r + w4[wi] * alpha_neg;
+                    if (vec_on) {
+                        r = rn;
+                        a.out1[p] = rn;
+                    }
+                }
+                if (vec_on) rmax = smax(rmax, fabs(r));
+                const double t1 = r - Pw;
+                const double tt = t1 - Ps;
+                const double zz = tt * w3[wi];
+                double pw = w1[wi] * zz;
+                double pn = w2[wi] * zz;
+                if (PM) {
+                    pw = fl ? pw : 0.0;
+                    pn = fl ? pn : 0.0;
+                }
+                Pw = pw;
+                Pn = pn;
+                if (valid && (!PM || fl)) a.out0[p] = zz;
+                if (eout && lane == kStrip - 1 && valid) st_relaxed(eout + x, as_bits(pn));
+            } else if (KIND == SW_BWD) {
+                // applyPreconditioner backward loop 7:789-803 + dotProduct(_z, _r)
+                const int c = cols - 1 - t;
+                const int x = c - lane;
+                const bool valid = row_ok && x >= 0 && x < W;
+                const long p = gbase + (long)c * kStrip;
+                const double zn = wave_shl1(zs, wev[wi]);
+                const double t1 = w0[wi] - w1[wi] * zs;
+                const double tt = t1 - w2[wi] * zn;
+                const double zz = tt * w3[wi];
+                zs = PM ? (fl ? zz : 0.0) : zz;
+                if (valid) {
+                    double zdot = zz;
+                    if (!PM || fl)
+                        a.out0[p] = zz;
+                    else if (PM && !VM)
+                        zdot = w5[wi];  // the reference's _z there: A s (MODE 1) or the stale value (MODE 0)
+                    if (!VM || fl) acc += zdot * w4[wi];
+                }
+                if (eout && lane == 0 && valid) st_relaxed(eout + x, as_bits(zs));
+            } else {
+                // buildPreconditioner 7:744-767
+                const int x = t - lane;
+                const bool valid = row_ok && x >= 0 && x < W;
+                const double Ns = wave_shr1(Pn, wev[wi]);
+                const double ad = w0[wi];
+                double e = ad;
+                e = e - Pw;
+                e = e - Ns;
+                if (e < sigma * ad) e = ad;
+                const double pcv = 1.0 / sqrt(e);
+                const double px = w1[wi] * pcv;
+                const double py = w2[wi] * pcv;
+                const double eo = px * px + tau * px * py;
+                const double no = py * py + tau * px * py;
+                const bool on = valid && fl;
+                Pw = on ? eo : 0.0;
+                Pn = on ? no : 0.0;
+                if (on) {
+                    const long p = gbase + (long)t * kStrip;
+                    a.out0[p] = pcv;
+                    a.out1[p] = px;
+                    a.out2[p] = py;
+                }
+                if (eout && lane == kStrip - 1 && valid) st_relaxed(eout + x, as_bits(Pn));
             }
-            if (eout && l == kStrip - 1 && valid) st_relaxed(eout + x, as_bits(En));
         }
+        // every read of slot b has been consumed by now: hand it back
+        __hip_atomic_store(&hdr->consumed, b + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    }
+    if (KIND == SW_FWD) {
+        rmax = wave_max(rmax);
+        if (lane == 0) atomicMax((unsigned long long*)&st->max_err, (unsigned long long)as_bits(rmax));
+    } else if (KIND == SW_BWD) {
+        acc = wave_sum(acc);
+        if (lane == 0) a.partial[j] = acc;
     }
 }
 
@@ -550,62 +662,76 @@ static VecArgs vec_args(Solver* s) {
     return a;
 }
 
-static SweepArgs sweep_args(Solver* s, int which_edge) {
-    SweepArgs a;
+static StreamArgs stream_args(Solver* s, int kind) {
+    StreamArgs a;
+    memset(&a, 0, sizeof a);
     a.sk = s->sk;
-    a.r = s->r; a.q = s->qv; a.y = s->yv; a.z = s->z;
-    a.cx = s->cx; a.cy = s->cy; a.pc = s->precon;
     a.flag = s->dflag;
-    a.edge = s->edge + (long)which_edge * s->sk.nstrips * s->W;
+    a.edge = s->edge + (long)kind * s->sk.nstrips * s->W;
     a.partial = s->partial;
     a.st = s->st;
-    a.mode = 0;
+    a.timeout = &s->st->timeout;
+    a.tau = s->cfg.tau;
+    a.sigma = s->cfg.sigma;
     return a;
+}
+
+template <int KIND, bool PM, bool VM, int MODE, int NA>
+static int launch_stream(Solver* s, StreamArgs a) {
+    a.na = NA;
+    a.slot_bytes = NA * 4096 + 512;
+    size_t fixed = sizeof(RingHdr) + kERing * sizeof(double);
+    int nb = (int)((s->lds_bytes - fixed) / a.slot_bytes);
+    if (nb > 12) nb = 12;
+    if (nb < 2) {
+        s->err = "not enough LDS for the sweep ring";
+        return IFL_ERR_UNSUPPORTED;
+    }
+    a.nb = nb;
+    size_t shmem = fixed + (size_t)nb * a.slot_bytes;
+    static bool attr_set = false;
+    if (!attr_set) {
+        IFL_HIP(hipFuncSetAttribute((const void*)stream_sweep_kernel<KIND, PM, VM, MODE, NA>,
+                                    hipFuncAttributeMaxDynamicSharedMemorySize, (int)shmem));
+        attr_set = true;
+    }
+    stream_sweep_kernel<KIND, PM, VM, MODE, NA><<<sweep_blocks(s->sk.nstrips), 64 * kWaves, shmem, s->stream>>>(a);
+    IFL_LAUNCH_CHECK();
+    return IFL_OK;
 }
 
 template <int MODE>
 static int launch_fwd(Solver* s) {
-    SweepArgs a = sweep_args(s, 0);
-    int nb = sweep_blocks(s->sk.nstrips);
-    if (s->level == 3)
-        fwd_sweep_kernel<false, false, MODE><<<nb, 64, 0, s->stream>>>(a);
-    else if (s->level <= 5)
-        fwd_sweep_kernel<true, false, MODE><<<nb, 64, 0, s->stream>>>(a);
-    else
-        fwd_sweep_kernel<true, true, MODE><<<nb, 64, 0, s->stream>>>(a);
-    IFL_LAUNCH_CHECK();
-    return IFL_OK;
+    StreamArgs a = stream_args(s, SW_FWD);
+    a.in[0] = s->r; a.in[1] = s->cx; a.in[2] = s->cy; a.in[3] = s->precon; a.in[4] = s->qv;
+    a.out0 = s->yv;
+    a.out1 = s->r;
+    constexpr int NA = MODE == 1 ? 5 : 4;
+    if (s->level == 3) return launch_stream<SW_FWD, false, false, MODE, NA>(s, a);
+    if (s->level <= 5) return launch_stream<SW_FWD, true, false, MODE, NA>(s, a);
+    return launch_stream<SW_FWD, true, true, MODE, NA>(s, a);
 }
 
 template <int MODE>
 static int launch_bwd(Solver* s) {
-    SweepArgs a = sweep_args(s, 1);
-    int nb = sweep_blocks(s->sk.nstrips);
-    if (s->level == 3)
-        bwd_sweep_kernel<false, false, MODE><<<nb, 64, 0, s->stream>>>(a);
-    else if (s->level <= 5)
-        bwd_sweep_kernel<true, false, MODE><<<nb, 64, 0, s->stream>>>(a);
-    else
-        bwd_sweep_kernel<true, true, MODE><<<nb, 64, 0, s->stream>>>(a);
-    IFL_LAUNCH_CHECK();
-    return IFL_OK;
+    StreamArgs a = stream_args(s, SW_BWD);
+    a.in[0] = s->yv; a.in[1] = s->cx; a.in[2] = s->cy; a.in[3] = s->precon; a.in[4] = s->r;
+    a.in[5] = MODE == 1 ? s->qv : s->z;  // the reference's _z at non-FLUID cells (levels 4-5 dot)
+    a.out0 = s->z;
+    if (s->level == 3) return launch_stream<SW_BWD, false, false, MODE, 5>(s, a);
+    if (s->level <= 5) return launch_stream<SW_BWD, true, false, MODE, 6>(s, a);
+    return launch_stream<SW_BWD, true, true, MODE, 5>(s, a);
 }
 
 static long skew_n(Solver* s) { return (long)s->sk.nstrips * s->sk.strip_sz; }
 
 int k_build_precon(Solver* s) {
-    PreconArgs a;
-    a.sk = s->sk;
-    a.adiag = s->adiag; a.apx = s->aplusx; a.apy = s->aplusy;
-    a.pc = s->precon; a.cx = s->cx; a.cy = s->cy;
-    a.flag = s->dflag;
-    a.edge = s->edge + 2L * s->sk.nstrips * s->W;
-    a.timeout = &s->st->timeout;
-    a.tau = s->cfg.tau;
-    a.sigma = s->cfg.sigma;
-    precon_kernel<<<sweep_blocks(s->sk.nstrips), 64, 0, s->stream>>>(a);
-    IFL_LAUNCH_CHECK();
-    return IFL_OK;
+    StreamArgs a = stream_args(s, SW_PRECON);
+    a.in[0] = s->adiag; a.in[1] = s->aplusx; a.in[2] = s->aplusy;
+    a.out0 = s->precon;
+    a.out1 = s->cx;
+    a.out2 = s->cy;
+    return launch_stream<SW_PRECON, true, false, 0, 3>(s, a);
 }
 
 int k_apply_precon(Solver* s) {
