@@ -513,10 +513,7 @@ int ifl_stage(ifl_solver* h, int stage, int iarg, double darg, ifl_solve_stats* 
         flip_q(s, iarg);
         break;
     case IFL_STAGE_BUILD_RHS: rc = k_build_rhs(s); break;
-    case IFL_STAGE_BUILD_PRESSURE:
-        if (lv >= 7) rc = k_build_pressure(s, darg);
-        else rc = k_build_pressure(s, darg);
-        break;
+    case IFL_STAGE_BUILD_PRESSURE: rc = k_build_pressure(s, darg); break;
     case IFL_STAGE_BUILD_HEAT:
         if (lv < 6) return IFL_ERR_UNSUPPORTED;
         rc = k_build_heat(s, darg);
@@ -593,6 +590,27 @@ int ifl_profile_read(const ifl_solver* h, int kernel, int64_t* launches, double*
     if (iterations) *iterations = p.iterations;
     if (cell_iterations) *cell_iterations = p.cell_iterations;
     return IFL_OK;
+}
+
+int ifl_debug_trace_sweep(ifl_solver* h, int kind, int64_t* host_out, size_t count, int* blocks_per_strip) {
+    if (!h || kind < 0 || kind > 2 || !host_out) return IFL_ERR_ARG;
+    Solver* s = &h->s;
+    IFL_HIP(hipSetDevice(s->device));
+    int nblk = s->sk.cols / 8;
+    int nb = 8 * ((s->sk.nstrips + 7) / 8);
+    size_t need = (size_t)nb * nblk * 3;
+    if (blocks_per_strip) *blocks_per_strip = nblk;
+    if (count < need) return set_err(s, IFL_ERR_ARG, "trace buffer too small");
+    long long* d = nullptr;
+    IFL_HIP(hipMalloc((void**)&d, need * sizeof(long long)));
+    IFL_HIP(hipMemsetAsync(d, 0, need * sizeof(long long), s->stream));
+    int rc = k_trace_sweep(s, kind, d);
+    if (rc == IFL_OK) {
+        IFL_HIP(hipMemcpyAsync(host_out, d, need * sizeof(long long), hipMemcpyDeviceToHost, s->stream));
+        IFL_HIP(hipStreamSynchronize(s->stream));
+    }
+    (void)hipFree(d);
+    return rc;
 }
 
 int ifl_synchronize(ifl_solver* h) {

@@ -141,6 +141,7 @@ int k_matvec(Solver* s);             // z = A s, the stage form
 int k_dot(Solver* s, int which, double* out);  // 0: z.r  1: z.s
 int k_inf_norm(Solver* s, double* out);
 int k_project(Solver* s, int limit, ifl_solve_stats* stats);
+int k_trace_sweep(Solver* s, int kind, long long* dev_trace);
 
 // ---- launchers (k_flip.hip) ----
 int flip_init_particles(Solver* s);

@@ -27,9 +27,7 @@
 
 namespace ifl {
 
-constexpr int kU = 8;  // sweep steps per unrolled block
 constexpr uint64_t kSentinel = 0x7FF4DEADBEEF1234ull;  // signalling NaN
-constexpr long kSpinLimit = 1L << 18;  // ~0.3 s of polling; a legitimate wait is at most ~1 ms
 
 IFL_DEV uint64_t ld_relaxed(const double* p) {
     return __hip_atomic_load((const uint64_t*)p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -50,31 +48,6 @@ IFL_DEV int strip_of_block(int b, int nstrips) {
 }
 static int sweep_blocks(int nstrips) { return 8 * ((nstrips + 7) / 8); }
 
-// Lanes [0, kU) fetch the kU edge values of one block; each value is reset
-// to the sentinel once read, ready for the next sweep.
-IFL_DEV double edge_fetch(double* e, int x, bool active, int* timeout) {
-    uint64_t v = active ? ld_relaxed(e + x) : 0;
-    long spins = 0;
-    if (__hip_atomic_load(timeout, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) spins = kSpinLimit;
-    while (__any(active && v == kSentinel)) {
-        if (++spins > kSpinLimit) {
-            if (active) atomicOr(timeout, 1);
-            v = 0;
-            break;
-        }
-        __builtin_amdgcn_s_sleep(1);
-        if (active && v == kSentinel) v = ld_relaxed(e + x);
-    }
-    if (active) st_relaxed(e + x, kSentinel);
-    return as_double(v);
-}
-
-IFL_DEV double lane_value(double v, int lane) {
-    int lo = __builtin_amdgcn_readlane(__double2loint(v), lane);
-    int hi = __builtin_amdgcn_readlane(__double2hiint(v), lane);
-    return __hiloint2double(hi, lo);
-}
-
 
 // ---------------------------------------------------------------- streamed sweeps
 // A sweep step needs ~2.5 KB of fresh data per strip while one wave can keep
@@ -85,14 +58,13 @@ IFL_DEV double lane_value(double v, int lane) {
 //                 per round trip into a small LDS ring;
 //   waves 2..6    loaders, each streaming every kLoaders-th block of 8 steps
 //                 (4 KB contiguous per array in the skewed layout) from HBM
-//                 into the LDS ring, with kDepth blocks in flight.
+//                 into the LDS ring, up to two blocks in flight each.
 // Ring slots are recycled through counters in LDS (acquire/release at
 // workgroup scope). Every wait is bounded; a timeout raises `abort`, which
 // every wave checks, so the workgroup always drains.
 constexpr int kBlk = 8;
 constexpr int kLoaders = 5;
 constexpr int kWaves = 2 + kLoaders;
-constexpr int kDepth = 2;
 constexpr int kERing = 512;  // edge ring, in steps
 constexpr int kMaxArrays = 6;
 constexpr long kLdsSpin = 1L << 22;
@@ -122,6 +94,7 @@ struct StreamArgs {
     double tau, sigma;
     int nb;          // ring depth in blocks
     int slot_bytes;  // na*4096 + 512
+    long long* trace;  // diagnostics: per strip and block {wait begin, wait end, published}, or null
 };
 
 IFL_DEV int lds_acq(int* p) { return __hip_atomic_load(p, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_WORKGROUP); }
@@ -225,6 +198,7 @@ IFL_DEV void loader_wave(const StreamArgs& a, char* ring, RingHdr* hdr, int k, i
             IFL_WAIT_VMCNT(0);
         }
         lds_st_raw(&hdr->ready[k], b + 1);
+        if (a.trace && lane == 0) a.trace[((long)blockIdx.x * nblk + b) * 3 + 2] = __builtin_amdgcn_s_memrealtime();
         if (has_next && !issued) {
             if (!lds_wait_gt_raw(&hdr->consumed, bn - a.nb, hdr, a.timeout)) return;
             issue(bn);
@@ -341,7 +315,9 @@ __global__ __launch_bounds__(64 * kWaves) void stream_sweep_kernel(StreamArgs a)
     rd(1, 1);
     rd(2, 2);
     for (int b = 0; b < nblk; b++) {
+        if (a.trace && lane == 0) a.trace[((long)blockIdx.x * nblk + b) * 3 + 0] = __builtin_amdgcn_s_memrealtime();
         if (b + 1 < nblk && !ready(b + 1)) return;
+        if (a.trace && lane == 0) a.trace[((long)blockIdx.x * nblk + b) * 3 + 1] = __builtin_amdgcn_s_memrealtime();
         const int T = b * kBlk;
 #pragma unroll
         for (int k = 0; k < kBlk; k++) {
@@ -445,7 +421,8 @@ struct VecArgs {
     const uint8_t* flag;
     double* partial;
     PcgState* st;
-    int vm;
+    int pm;  // levels 4+: the preconditioner writes _z at FLUID cells only
+    int vm;  // levels 6+: vector operations are masked to FLUID cells
 };
 
 IFL_DEV bool skew_cell(const Skew& sk, long i, int& x, int& y) {
@@ -460,6 +437,9 @@ IFL_DEV bool skew_cell(const Skew& sk, long i, int& x, int& y) {
 // A1: the tail of the previous iteration, scaledAdd(_p, _p, _s, alpha) 7:869
 // and scaledAdd(_s, _z, _s, sigmaNew/sigma) 7:881. mode 1 runs only the s
 // update (the budget-exhausted tail), mode 2 only the p update.
+// The reference's _z at a non-FLUID cell is still A s from the matrix-vector
+// product (applyPreconditioner skips the cell, 4:766-796); this product keeps
+// A s in q, so levels 4-5 (unmasked scaledAdd) read it from there.
 __global__ void vec_update_kernel(VecArgs a, int mode, int check_done) {
     PcgState* st = a.st;
     if (check_done && st->done) return;
@@ -468,10 +448,22 @@ __global__ void vec_update_kernel(VecArgs a, int mode, int check_done) {
     if (i >= n) return;
     int x, y;
     if (!skew_cell(a.sk, i, x, y)) return;
-    if (a.vm && !a.flag[i]) return;
+    const bool fl = a.flag[i] != 0;
+    if (a.vm && !fl) return;
     double s = a.s[i];
     if (mode != 1) a.p[i] = a.p[i] + s * st->alpha;
-    if (mode != 2) a.s[i] = a.z[i] + s * st->beta;
+    if (mode != 2) a.s[i] = (a.pm && !fl ? a.q[i] : a.z[i]) + s * st->beta;
+}
+
+// End of a solve that ran out of its budget: the reference's _z holds A s
+// at every non-FLUID cell (levels 4+), the preconditioned residual elsewhere.
+__global__ void z_nonfluid_kernel(VecArgs a) {
+    long n = (long)a.sk.nstrips * a.sk.strip_sz;
+    long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    int x, y;
+    if (!skew_cell(a.sk, i, x, y) || a.flag[i]) return;
+    a.z[i] = a.q[i];
 }
 
 // A2: matrixVectorProduct 7:816-833 (unmasked at every level; the four
@@ -658,6 +650,7 @@ static VecArgs vec_args(Solver* s) {
     a.flag = s->dflag;
     a.partial = s->partial;
     a.st = s->st;
+    a.pm = s->level >= 4;
     a.vm = s->level >= 6;
     return a;
 }
@@ -665,6 +658,7 @@ static VecArgs vec_args(Solver* s) {
 static StreamArgs stream_args(Solver* s, int kind) {
     StreamArgs a;
     memset(&a, 0, sizeof a);
+    a.trace = nullptr;
     a.sk = s->sk;
     a.flag = s->dflag;
     a.edge = s->edge + (long)kind * s->sk.nstrips * s->W;
@@ -732,6 +726,31 @@ int k_build_precon(Solver* s) {
     a.out1 = s->cx;
     a.out2 = s->cy;
     return launch_stream<SW_PRECON, true, false, 0, 3>(s, a);
+}
+
+// Diagnostics: one sweep of `kind` on the current state with the compute
+// wave's waits and the loaders' publications time-stamped (100 MHz clock).
+int k_trace_sweep(Solver* s, int kind, long long* dev_trace) {
+    StreamArgs a;
+    if (kind == SW_PRECON) {
+        a = stream_args(s, SW_PRECON);
+        a.in[0] = s->adiag; a.in[1] = s->aplusx; a.in[2] = s->aplusy;
+        a.out0 = s->precon; a.out1 = s->cx; a.out2 = s->cy;
+        a.trace = dev_trace;
+        return launch_stream<SW_PRECON, true, false, 0, 3>(s, a);
+    }
+    if (kind == SW_FWD) {
+        a = stream_args(s, SW_FWD);
+        a.in[0] = s->r; a.in[1] = s->cx; a.in[2] = s->cy; a.in[3] = s->precon; a.in[4] = s->qv;
+        a.out0 = s->yv; a.out1 = s->r;
+        a.trace = dev_trace;
+        return s->level >= 6 ? launch_stream<SW_FWD, true, true, 0, 4>(s, a) : launch_stream<SW_FWD, true, false, 0, 4>(s, a);
+    }
+    a = stream_args(s, SW_BWD);
+    a.in[0] = s->yv; a.in[1] = s->cx; a.in[2] = s->cy; a.in[3] = s->precon; a.in[4] = s->r; a.in[5] = s->z;
+    a.out0 = s->z;
+    a.trace = dev_trace;
+    return s->level >= 6 ? launch_stream<SW_BWD, true, true, 0, 5>(s, a) : launch_stream<SW_BWD, true, false, 0, 6>(s, a);
 }
 
 int k_apply_precon(Solver* s) {
@@ -840,6 +859,10 @@ int k_project(Solver* s, int limit, ifl_solve_stats* stats) {
     } else if (h.status == IFL_SOLVE_BUDGET && limit > 0) {
         vec_update_kernel<<<grid_blocks(n, 256), 256, 0, s->stream>>>(a, 0, 0);
         IFL_LAUNCH_CHECK();
+        if (a.pm) {
+            z_nonfluid_kernel<<<grid_blocks(n, 256), 256, 0, s->stream>>>(a);
+            IFL_LAUNCH_CHECK();
+        }
     }
     if (s->profiling) {
         IFL_HIP(hipEventRecord(s->ev1, s->stream));

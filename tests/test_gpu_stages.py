@@ -1,0 +1,93 @@
+"""Teacher-forced parity of every stage of FluidSolver::update() on the GPU.
+
+The oracle runs a scene for a few steps and then walks through the stages of
+one update() in the reference's order (3:404-441, 4:862-894, 6:987-1026,
+7:987-1030). Before each stage the oracle's whole state is uploaded into the
+product, both run the stage, and every state array (and the stage's scalar
+result) must agree bit for bit. Dot products use the reference's summation
+order (dot_order = SEQUENTIAL) so that they are bit-exact too.
+
+project() is also run with a 3-iteration budget so that the budget-exit state
+(_z holding A s at non-FLUID cells, the deferred s update) is covered.
+"""
+import pytest
+
+import oracle_lib as ol
+from gpu_util import compare_fields, copy_state, make_product
+
+pytestmark = pytest.mark.gpu
+
+MULTI = [(ol.BODY_BOX, 0.5, 0.6, 0.3, 0.1, 0.7, 0.0, 0.0, 0.0),
+         (ol.BODY_SPHERE, 0.3, 0.35, 0.15, 0.0, 0.0, 0.0, 0.0, 0.0),
+         (ol.BODY_BOX, 0.7, 0.3, 0.1, 0.12, -0.4, 0.0, 0.0, 0.0)]
+
+
+def stage_sequence(level, dt):
+    """(stage, iarg, darg, advance): the stages of one update() in order.
+    advance=False marks a probe: it is compared, then the oracle's state is
+    restored, so that the chain continues from the state update() would see."""
+    qs = [ol.Q_D, ol.Q_U, ol.Q_V] + ([ol.Q_T] if level >= 6 else [])
+    seq = []
+    if level >= 4:
+        seq += [("FILL_SOLID", q, 0.0, True) for q in qs]
+    if level >= 6:
+        seq += [("COPY_T_TO_R", 0, 0.0, True), ("BUILD_HEAT", 0, dt, True), ("BUILD_PRECON", 0, 0.0, True),
+                ("PROJECT", 3, 0.0, False), ("PROJECT", 2000, 0.0, True), ("COPY_P_TO_T", 0, 0.0, True)]
+    seq += [("BUILD_RHS", 0, 0.0, True)]
+    if level >= 7:
+        seq += [("COMPUTE_DENSITIES", 0, 0.0, True)]
+    seq += [("BUILD_PRESSURE", 0, dt, True), ("BUILD_PRECON", 0, 0.0, True),
+            ("APPLY_PRECON", 0, 0.0, False), ("DOT", 0, 0.0, False), ("INF_NORM", 0, 0.0, False),
+            ("MATVEC", 0, 0.0, False), ("DOT", 1, 0.0, False),
+            ("PROJECT", 0, 0.0, False), ("PROJECT", 1, 0.0, False), ("PROJECT", 3, 0.0, False),
+            ("PROJECT", 600, 0.0, True), ("APPLY_PRESSURE", 0, dt, True)]
+    if level >= 6:
+        seq += [("ADD_BUOYANCY", 0, dt, True)]
+    if level >= 4:
+        seq += [("SET_BOUNDARY", 0, 0.0, True)]
+    seq += [("ADVECT", q, dt, True) for q in qs]
+    if level >= 4:
+        seq += [("EXTRAPOLATE", q, 0.0, True) for q in qs]
+    seq += [("FLIP", q, 0.0, True) for q in qs]
+    return seq
+
+
+CASES = [
+    (3, 48, 40, None, 3),
+    (4, 72, 40, MULTI, 3),
+    (5, 72, 40, MULTI, 3),
+    (6, 72, 40, MULTI, 3),
+    (6, 64, 64, None, 4),
+    (7, 72, 40, MULTI, 3),
+    (7, 130, 70, MULTI, 2),
+]
+
+
+@pytest.mark.parametrize("level,w,h,bodies,warm", CASES)
+def test_stages_teacher_forced(level, w, h, bodies, warm):
+    sc = dict(ol.REF_SCENES[level])
+    if bodies is not None:
+        sc["bodies"] = bodies
+    orc = ol.OracleSolver(level, w, h, sc["params"], sc["bodies"], fresh=True)
+    for _ in range(warm):
+        if sc["inflow"] is not None:
+            orc.add_inflow(*sc["inflow"])
+        orc.update(sc["dt"])
+    gpu = make_product(level, w, h, sc["params"], sc["bodies"], dot_order=1)
+    gpu._push_bodies()
+    fields = ol.state_fields(level)
+    for name, iarg, darg, advance in stage_sequence(level, sc["dt"]):
+        copy_state(orc, gpu, fields)
+        saved = None if advance else {f: orc.field(f) for f in fields}
+        want = orc.stage(name, iarg, darg)
+        got = gpu.stage(name, iarg, darg)
+        where = f"level {level} {w}x{h} stage {name}({iarg})"
+        assert got == want or (got != got and want != want), f"{where}: result {got!r} vs {want!r}"
+        bad = compare_fields(orc, gpu, fields, where=where)
+        assert not bad, "\n".join(bad[:8])
+        if saved is not None:
+            for f, v in saved.items():
+                if v is not None:
+                    orc.set_field(f, v)
+    gpu.close()
+    orc.close()
