@@ -211,9 +211,11 @@ int ifl_profile_reset(ifl_solver* s);
 int ifl_synchronize(ifl_solver* s);
 
 /* ---- diagnostics: one MIC(0) sweep (kind 0 forward, 1 backward, 2 factor
- *      build) on the current state, time-stamped per workgroup and 8-step
- *      block: {compute wave begins waiting, data ready, loader published}
- *      in 100 MHz ticks. count >= 8*ceil(nstrips/8) * blocks_per_strip * 3. */
+ *      build) on the current state, time-stamped per 64-row strip and
+ *      8-step block, in 100 MHz ticks: {block begins, its incoming edges
+ *      are ready, first half computed, next block's operands ready, second
+ *      half computed, results written}. count >= nstrips * blocks_per_strip * 6,
+ *      laid out [strip][block][6]. */
 int ifl_debug_trace_sweep(ifl_solver* s, int kind, int64_t* host_out, size_t count, int* blocks_per_strip);
 
 #ifdef __cplusplus

@@ -111,7 +111,7 @@ void free_all(Solver* s) {
     }
     F(s->udens); F(s->vdens);
     F(s->r); F(s->p); F(s->z); F(s->s); F(s->precon); F(s->adiag); F(s->aplusx); F(s->aplusy);
-    F(s->cx); F(s->cy); F(s->yv); F(s->qv); F(s->dflag); F(s->edge); F(s->partial); F(s->st);
+    F(s->cx); F(s->cy); F(s->yv); F(s->qv); F(s->dflag); F(s->fmask); F(s->edge); F(s->partial); F(s->st);
     F(s->stage_buf); F(s->front); F(s->front_cnt); F(s->bodies_d);
     F(s->posx); F(s->posy);
     for (int t = 0; t < 4; t++) F(s->prop[t]);
@@ -130,6 +130,9 @@ int init_solver(Solver* s) {
     int lds = 0;
     IFL_HIP(hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, s->device));
     s->lds_bytes = (size_t)lds;
+    int cus = 0;
+    IFL_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, s->device));
+    s->num_cus = cus > 0 ? cus : 1;
     int W = s->W, H = s->H, lv = s->level;
     // FluidSolver ctor 7:953-958: d, t at cell centres, u, v on faces
     const int qw[4] = {W, W, W + 1, W};
@@ -172,6 +175,8 @@ int init_solver(Solver* s) {
                        &s->cx, &s->cy, &s->yv, &s->qv};
     for (double** a : arrs) IFL_TRY(dalloc(s, a, n));
     IFL_TRY(balloc(s, &s->dflag, n));
+    IFL_HIP(hipMalloc((void**)&s->fmask, (size_t)sk.nstrips * sk.cols * sizeof(uint64_t)));
+    IFL_HIP(hipMemsetAsync(s->fmask, 0, (size_t)sk.nstrips * sk.cols * sizeof(uint64_t), s->stream));
     long ne = 3L * sk.nstrips * W;
     IFL_HIP(hipMalloc((void**)&s->edge, ne * sizeof(double)));
     fill_u64_kernel<<<grid_blocks(ne, 256), 256, 0, s->stream>>>((uint64_t*)s->edge, ne, 0x7FF4DEADBEEF1234ull);
@@ -597,8 +602,7 @@ int ifl_debug_trace_sweep(ifl_solver* h, int kind, int64_t* host_out, size_t cou
     Solver* s = &h->s;
     IFL_HIP(hipSetDevice(s->device));
     int nblk = s->sk.cols / 8;
-    int nb = 8 * ((s->sk.nstrips + 7) / 8);
-    size_t need = (size_t)nb * nblk * 3;
+    size_t need = (size_t)s->sk.nstrips * nblk * kTraceStamps;
     if (blocks_per_strip) *blocks_per_strip = nblk;
     if (count < need) return set_err(s, IFL_ERR_ARG, "trace buffer too small");
     long long* d = nullptr;

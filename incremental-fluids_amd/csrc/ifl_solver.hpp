@@ -59,6 +59,7 @@ struct Solver {
     double *yv = nullptr;                 // forward-sweep output
     double *qv = nullptr;                 // A*s
     uint8_t* dflag = nullptr;             // 1 where the d-cell is FLUID (and inside the grid)
+    uint64_t* fmask = nullptr;            // dflag as one 64-lane mask per strip column (nstrips*cols)
     double* edge = nullptr;               // inter-strip hand-off, 2 buffers of nstrips*W
     double* partial = nullptr;            // per-block partial reductions
     int npartial = 0;
@@ -90,6 +91,7 @@ struct Solver {
 
     int border_solid = 0;
     size_t lds_bytes = 65536;  // dynamic LDS available to one workgroup
+    int num_cus = 1;           // compute units: the strips of one sweep launch that can be resident at once
 };
 
 // error helpers
@@ -141,6 +143,8 @@ int k_matvec(Solver* s);             // z = A s, the stage form
 int k_dot(Solver* s, int which, double* out);  // 0: z.r  1: z.s
 int k_inf_norm(Solver* s, double* out);
 int k_project(Solver* s, int limit, ifl_solve_stats* stats);
+// sweep diagnostics: time stamps per strip and 8-step block
+constexpr int kTraceStamps = 6;
 int k_trace_sweep(Solver* s, int kind, long long* dev_trace);
 
 // ---- launchers (k_flip.hip) ----

@@ -676,9 +676,21 @@ __global__ void dflag_kernel(const uint8_t* __restrict__ cell, uint8_t* __restri
     dflag[sk.pos(i % sk.W, i / sk.W)] = (all_fluid || cell[i] == IFL_CELL_FLUID) ? 1 : 0;
 }
 
+// One wave per strip column: the column's 64 flags as a lane mask, the
+// form in which the sweeps select per lane without vector work.
+__global__ __launch_bounds__(256) void fmask_kernel(const uint8_t* __restrict__ dflag, uint64_t* __restrict__ fmask, long n) {
+    long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    bool f = i < n && dflag[i] != 0;
+    uint64_t m = __ballot(f);
+    if ((threadIdx.x & 63) == 0 && i < n) fmask[i >> 6] = m;
+}
+
 int k_update_dflag(Solver* s) {
     dflag_kernel<<<grid_blocks((long)s->W * s->H, 256), 256, 0, s->stream>>>(s->q[IFL_Q_D].cell, s->dflag, s->sk,
                                                                              s->level == 3 ? 1 : 0);
+    IFL_LAUNCH_CHECK();
+    long n = (long)s->sk.nstrips * s->sk.strip_sz;
+    fmask_kernel<<<grid_blocks(n, 256), 256, 0, s->stream>>>(s->dflag, s->fmask, n);
     IFL_LAUNCH_CHECK();
     return IFL_OK;
 }

@@ -40,73 +40,101 @@ IFL_DEV uint64_t as_bits(double v) { return (uint64_t)__double_as_longlong(v); }
 
 // Strips are assigned so that consecutive strips share an XCD (workgroups
 // are dealt round-robin over the 8 XCDs): the edge hand-off then stays in
-// one L2.
-IFL_DEV int strip_of_block(int b, int nstrips) {
-    int per = (nstrips + 7) >> 3;
+// one L2. `n` strips from `first` are launched together.
+IFL_DEV int strip_of_block(int b, int first, int n) {
+    int per = (n + 7) >> 3;
     int s = (b & 7) * per + (b >> 3);
-    return ((b >> 3) < per && s < nstrips) ? s : -1;
+    return ((b >> 3) < per && s < n) ? first + s : -1;
 }
-static int sweep_blocks(int nstrips) { return 8 * ((nstrips + 7) / 8); }
+static int sweep_blocks(int n) { return 8 * ((n + 7) / 8); }
 
 
 // ---------------------------------------------------------------- streamed sweeps
-// A sweep step needs ~2.5 KB of fresh data per strip while one wave can keep
-// only a few hundred nanoseconds of loads in flight, far less than the HBM
-// latency. Each strip therefore runs as a workgroup of
-//   wave 0        the recurrence, reading its operands from an LDS ring;
-//   wave 1        the edge poller, fetching 64 upcoming inter-strip values
-//                 per round trip into a small LDS ring;
-//   waves 2..6    loaders, each streaming every kLoaders-th block of 8 steps
+// One strip runs as a workgroup of eight waves:
+//   wave 0        the recurrence. It only does what the dependency chain
+//                 needs: operands come from an LDS ring into registers half a
+//                 block ahead, FLUID masks come as 64-lane masks through scalar
+//                 loads, and results go to an LDS output ring. A single wave
+//                 issues one instruction at a time, so everything that is not
+//                 on the chain lives in the other waves;
+//   wave 1        the edge poller: fetches upcoming inter-strip values from
+//                 global memory, 64 steps per round trip, into an LDS ring;
+//   wave 2        the writer: stores the results of each finished block to
+//                 HBM, hands the edge row to the neighbouring strip, and
+//                 forms the reductions (max|r|, the z.r partial);
+//   waves 3..7    loaders: each streams every kLoaders-th block of 8 steps
 //                 (4 KB contiguous per array in the skewed layout) from HBM
-//                 into the LDS ring, up to two blocks in flight each.
-// Ring slots are recycled through counters in LDS (acquire/release at
-// workgroup scope). Every wait is bounded; a timeout raises `abort`, which
+//                 straight into the LDS ring.
+// Slot life cycle: loader fills -> wave 0 computes -> writer drains -> free.
+// Counters in LDS order the hand-offs (LDS instructions of one workgroup are
+// executed in order). Every wait is bounded; a timeout raises `abort`, which
 // every wave checks, so the workgroup always drains.
+//
+// The recurrence masks each cell's result to +0 at non-FLUID cells (zm) and
+// the factor sweep stores cx = cy = +0 there, so a neighbour the reference
+// skips contributes exactly cx*(+0) = +0, the same as a skipped term.
 constexpr int kBlk = 8;
+constexpr int kHalf = 4;
 constexpr int kLoaders = 5;
-constexpr int kWaves = 2 + kLoaders;
+constexpr int kWaves = 3 + kLoaders;
 constexpr int kERing = 512;  // edge ring, in steps
-constexpr int kMaxArrays = 6;
+constexpr int kMaxIn = 6;
+constexpr int kMaxOut = 3;
 constexpr long kLdsSpin = 1L << 22;
 
 enum { SW_FWD = 0, SW_BWD = 1, SW_PRECON = 2 };
 
 struct RingHdr {
-    int consumed;            // blocks the compute wave has finished with
-    int ready[kLoaders];     // per loader: 1 + last block published
-    int edge_ready;          // steps whose edge value is in the ring
+    int consumed;          // blocks whose slot is free again (writer)
+    int computed;          // blocks wave 0 has finished (wave 0)
+    int ready[kLoaders];   // per loader: 1 + last block published
+    int edge_ready;        // steps whose incoming edge value is in the edge ring
     int abort;
-    int pad[8];
+    int pad[7];
 };
 
 struct StreamArgs {
     Skew sk;
-    const double* in[kMaxArrays];
-    int na;
-    const uint8_t* flag;
-    double* edge;
-    double* out0;  // fwd: y   bwd: z   precon: precon
-    double* out1;  // fwd: r   precon: cx
-    double* out2;  // precon: cy
+    const double* in[kMaxIn];
+    const uint64_t* fmask;
+    double* edge;           // this sweep's edge buffers (nstrips * W)
+    double* out[kMaxOut];   // fwd: y, r   bwd: z   precon: precon, cx, cy
     double* partial;
     PcgState* st;
     int* timeout;
     double tau, sigma;
-    int nb;          // ring depth in blocks
-    int slot_bytes;  // na*4096 + 512
-    long long* trace;  // diagnostics: per strip and block {wait begin, wait end, published}, or null
+    int strip0, nlaunch;    // the strips of this launch
+    int nb;                 // ring depth in blocks
+    int slot_bytes;         // (inputs + outputs) * 4096
+    long long* trace;       // diagnostics: kTraceStamps stamps per strip and block, or null
 };
 
-IFL_DEV int lds_acq(int* p) { return __hip_atomic_load(p, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_WORKGROUP); }
-IFL_DEV void lds_rel(int* p, int v) { __hip_atomic_store(p, v, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_WORKGROUP); }
+// s_waitcnt with only vmcnt constrained (gfx9 encoding: vmcnt in bits 3:0
+// and 15:14, expcnt 6:4 and lgkmcnt 11:8 left at their maxima).
+#define IFL_WAIT_VMCNT(n) __builtin_amdgcn_s_waitcnt(((n) & 0xF) | (0x7 << 4) | (0xF << 8) | (((n) >> 4) << 14))
 
-// Spin until *p > bound (or abort); false when aborted.
-IFL_DEV bool lds_wait_gt(int* p, int bound, RingHdr* hdr, int* gtimeout) {
+typedef __attribute__((address_space(3))) char lds_char;
+
+// LDS counters are read and written through inline asm: the compiler would
+// otherwise wait for every in-flight LDS-DMA load (vmcnt(0)) before any LDS
+// instruction in the loaders, and it must not move ring accesses across a
+// counter access in any wave ("memory" clobber).
+IFL_DEV unsigned lds_off(const void* p) { return (unsigned)(size_t)(const __attribute__((address_space(3))) char*)p; }
+IFL_DEV int lds_ld(const int* p) {
+    int v;
+    asm volatile("ds_read_b32 %0, %1\n\ts_waitcnt lgkmcnt(0)" : "=v"(v) : "v"(lds_off(p)) : "memory");
+    return __builtin_amdgcn_readfirstlane(v);
+}
+IFL_DEV void lds_st(int* p, int v) {
+    asm volatile("ds_write_b32 %0, %1" : : "v"(lds_off(p)), "v"(v) : "memory");
+}
+// Spin until *p > bound; false on abort or timeout (which raises abort).
+IFL_DEV bool lds_wait_gt(const int* p, int bound, RingHdr* hdr, int* gtimeout) {
     long spins = 0;
-    while (lds_acq(p) <= bound) {
-        if (lds_acq(&hdr->abort)) return false;
+    while (lds_ld(p) <= bound) {
+        if (lds_ld(&hdr->abort)) return false;
         if (++spins > kLdsSpin) {
-            lds_rel(&hdr->abort, 1);
+            lds_st(&hdr->abort, 1);
             atomicOr(gtimeout, 1);
             return false;
         }
@@ -120,121 +148,137 @@ template <int KIND>
 IFL_DEV int block_col0(int b, int cols) {
     return KIND == SW_BWD ? cols - kBlk - b * kBlk : b * kBlk;
 }
-// the column lane-0 (fwd/precon) or lane-63 (bwd) edge value of step t belongs to
+// skewed column processed at step t, and its index inside its block
+template <int KIND>
+IFL_DEV int step_col(int t, int cols) {
+    return KIND == SW_BWD ? cols - 1 - t : t;
+}
+template <int KIND>
+IFL_DEV int step_in_block(int k) {
+    return KIND == SW_BWD ? kBlk - 1 - k : k;
+}
+// the column the incoming edge value of step t belongs to (lane 0 in the
+// forward direction, lane 63 backward)
 template <int KIND>
 IFL_DEV int edge_x(int t, int cols) {
     return KIND == SW_BWD ? cols - kStrip - t : t;
 }
 
-// s_waitcnt with only vmcnt constrained (gfx9 encoding: vmcnt in bits 3:0
-// and 15:14, expcnt 6:4 and lgkmcnt 11:8 left at their maxima).
-#define IFL_WAIT_VMCNT(n) __builtin_amdgcn_s_waitcnt(((n) & 0xF) | (0x7 << 4) | (0xF << 8) | (((n) >> 4) << 14))
+typedef double vec2d __attribute__((ext_vector_type(2)));
 
-typedef __attribute__((address_space(3))) char lds_char;
+// Per sweep: NG arrays streamed from HBM, NS of them kept in the LDS slot
+// (the forward update folds q into r on the way), NC read by wave 0, and
+// one output array written by wave 0.
+template <int KIND, int MODE, int NG>
+struct SweepShape {
+    static constexpr int NS = NG - (KIND == SW_FWD && MODE == 1 ? 1 : 0);
+    static constexpr int NC = KIND == SW_PRECON ? 3 : 4;
+    static constexpr int R = NG <= 4 ? 3 : 2;  // blocks in flight per loader, NG*16 VGPRs each
+    static constexpr int SLOT = (NS + 1) * 4096;
+};
 
-// LDS counter access for the loader waves, written as inline asm: the
-// compiler would otherwise wait for every in-flight LDS-DMA load (vmcnt(0))
-// before any LDS instruction, serialising the load pipeline.
-IFL_DEV unsigned lds_off(const void* p) { return (unsigned)(size_t)(const __attribute__((address_space(3))) char*)p; }
-IFL_DEV int lds_ld_raw(const int* p) {
-    int v;
-    asm volatile("ds_read_b32 %0, %1\n\ts_waitcnt lgkmcnt(0)" : "=v"(v) : "v"(lds_off(p)) : "memory");
-    return v;
-}
-IFL_DEV void lds_st_raw(int* p, int v) {
-    asm volatile("ds_write_b32 %0, %1\n\ts_waitcnt lgkmcnt(0)" : : "v"(lds_off(p)), "v"(v) : "memory");
-}
-IFL_DEV bool lds_wait_gt_raw(const int* p, int bound, RingHdr* hdr, int* gtimeout) {
-    long spins = 0;
-    while (lds_ld_raw(p) <= bound) {
-        if (lds_ld_raw(&hdr->abort)) return false;
-        if (++spins > kLdsSpin) {
-            lds_st_raw(&hdr->abort, 1);
-            atomicOr(gtimeout, 1);
-            return false;
-        }
-        __builtin_amdgcn_s_sleep(1);
-    }
-    return true;
+IFL_DEV bool lane_on(uint64_t m) { return __builtin_amdgcn_inverse_ballot_w64(m); }
+
+// The masks do not change during a sweep: reading them through the constant
+// address space lets wave-uniform reads become scalar loads.
+typedef const __attribute__((address_space(4))) uint64_t const_u64;
+IFL_DEV const const_u64* strip_masks(const StreamArgs& a, int j) {
+    return (const const_u64*)(uintptr_t)(a.fmask + (long)j * a.sk.cols);
 }
 
-template <int KIND, int NA>
-IFL_DEV void loader_wave(const StreamArgs& a, char* ring, RingHdr* hdr, int k, int lane, long sbase, int nblk) {
+// Each loader streams blocks k, k + kLoaders, ... . Up to R blocks are in
+// flight in its registers (global_load_dwordx4: lane i holds elements
+// 2i, 2i+1 of each 128-element run), so the prefetch depth does not depend
+// on free LDS slots; a block is written into its slot once the slot is free.
+// In the forward sweep with the residual update (MODE 1) the loader forms
+// r + q*(-alpha) (scaledAdd 7:870; at FLUID cells only from level 6 on) and
+// keeps only the result.
+template <int KIND, int MODE, int LV, int NG>
+IFL_DEV void loader_wave(const StreamArgs& a, char* ring, RingHdr* hdr, int k, int lane, long sbase, int nblk, int j) {
+    typedef SweepShape<KIND, MODE, NG> S;
+    constexpr int R = S::R, NL = NG * 4;
+    constexpr bool UPD = KIND == SW_FWD && MODE == 1;
     const int cols = a.sk.cols;
-    constexpr int kLoadsPerBlock = NA * 4 + 2;
-    // Loads land in LDS directly (global_load_lds_dwordx4: lane i fills bytes
-    // [16 i, 16 i + 16) of the 1 KB run at the wave-uniform LDS address).
-    auto issue = [&](int b) {
-        long off = sbase + (long)block_col0<KIND>(b, cols) * kStrip;
-        lds_char* slot = (lds_char*)(ring + (long)(b % a.nb) * a.slot_bytes);
-#pragma unroll
-        for (int ai = 0; ai < NA; ai++)
-#pragma unroll
-            for (int m = 0; m < 4; m++)
-                __builtin_amdgcn_global_load_lds((const void*)(a.in[ai] + off + m * 128 + lane * 2),
-                                                 (__attribute__((address_space(3))) void*)(slot + ai * 4096 + m * 1024), 16, 0, 0);
-#pragma unroll
-        for (int m = 0; m < 2; m++)
-            __builtin_amdgcn_global_load_lds((const void*)(a.flag + off + m * 256 + lane * 4),
-                                             (__attribute__((address_space(3))) void*)(slot + NA * 4096 + m * 256), 4, 0, 0);
-    };
     const int nmine = nblk > k ? (nblk - k + kLoaders - 1) / kLoaders : 0;
-    if (nmine == 0) return;
-    if (!lds_wait_gt_raw(&hdr->consumed, k - a.nb, hdr, a.timeout)) return;
-    issue(k);
-    for (int i = 0; i < nmine; i++) {
+    const double alpha_neg = UPD ? -a.st->alpha : 0.0;
+    const const_u64* fm = strip_masks(a, j);
+    vec2d b0[NL], b1[NL], b2[NL];
+    auto issue = [&](int i, vec2d* B) __attribute__((always_inline)) {
+        const long off = sbase + (long)block_col0<KIND>(k + i * kLoaders, cols) * kStrip + lane * 2;
+#pragma unroll
+        for (int ai = 0; ai < NG; ai++)
+#pragma unroll
+            for (int m = 0; m < 4; m++) B[ai * 4 + m] = *(const vec2d*)(a.in[ai] + off + m * 128);
+    };
+    // land block i (from B) in its slot, then reuse B for block i + R
+    auto stage = [&](int i, vec2d* B) __attribute__((always_inline)) -> bool {
+        if (i >= nmine) return true;
         const int b = k + i * kLoaders;
-        const int bn = b + kLoaders;
-        const bool has_next = i + 1 < nmine;
-        // Put the next block in flight only if its slot is already free:
-        // waiting for it here could wait on the compute wave, which may be
-        // waiting for block b.
-        bool issued = false;
-        if (has_next && lds_ld_raw(&hdr->consumed) > bn - a.nb) {
-            issue(bn);
-            issued = true;
-            IFL_WAIT_VMCNT(kLoadsPerBlock);  // block b has landed, bn may still be in flight
-        } else {
-            IFL_WAIT_VMCNT(0);
+        if (UPD) {
+            const int col0 = block_col0<KIND>(b, cols);
+#pragma unroll
+            for (int m = 0; m < 4; m++) {
+                const vec2d r = B[m];
+                vec2d rn = r + B[16 + m] * alpha_neg;
+                if (LV >= 6) {
+                    // element 2*lane + c of run m is step 2m + lane/32, row 2*(lane%32) + c
+                    const uint64_t w = lane < 32 ? fm[col0 + 2 * m] : fm[col0 + 2 * m + 1];
+                    const unsigned bits = (unsigned)(w >> (2 * (lane & 31)));
+                    rn.x = (bits & 1u) ? rn.x : r.x;
+                    rn.y = (bits & 2u) ? rn.y : r.y;
+                }
+                B[m] = rn;
+            }
         }
-        lds_st_raw(&hdr->ready[k], b + 1);
-        if (a.trace && lane == 0) a.trace[((long)blockIdx.x * nblk + b) * 3 + 2] = __builtin_amdgcn_s_memrealtime();
-        if (has_next && !issued) {
-            if (!lds_wait_gt_raw(&hdr->consumed, bn - a.nb, hdr, a.timeout)) return;
-            issue(bn);
-        }
+        if (!lds_wait_gt(&hdr->consumed, b - a.nb, hdr, a.timeout)) return false;
+        char* slot = ring + (long)(b % a.nb) * a.slot_bytes;
+#pragma unroll
+        for (int ai = 0; ai < S::NS; ai++)
+#pragma unroll
+            for (int m = 0; m < 4; m++) *(vec2d*)(slot + ai * 4096 + m * 1024 + lane * 16) = B[ai * 4 + m];
+        lds_st(&hdr->ready[k], b + 1);
+        if (i + R < nmine) issue(i + R, B);
+        return true;
+    };
+    if (nmine > 0) issue(0, b0);
+    if (R > 1 && nmine > 1) issue(1, b1);
+    if (R > 2 && nmine > 2) issue(2, b2);
+    for (int i = 0; i < nmine; i += R) {
+        if (!stage(i, b0)) return;
+        if (R > 1 && !stage(i + 1, b1)) return;
+        if (R > 2 && !stage(i + 2, b2)) return;
     }
-    IFL_WAIT_VMCNT(0);
 }
 
+// Strips without a neighbour on the input side get +0 edges.
 template <int KIND>
-IFL_DEV void poller_wave(const StreamArgs& a, double* ering, RingHdr* hdr, int lane) {
+IFL_DEV void poller_wave(const StreamArgs& a, const double* ein, double* ering, RingHdr* hdr, int lane) {
     const int cols = a.sk.cols, W = a.sk.W;
     int t0 = 0;
     long spins = 0;
     while (t0 < cols) {
-        if (lds_acq(&hdr->abort)) return;
-        // ring capacity: steps below consumed*kBlk are free
-        int cap = lds_acq(&hdr->consumed) * kBlk + kERing;
-        int t = t0 + lane;
-        int x = edge_x<KIND>(t, cols);
-        bool inwin = t < cols && t < cap;
-        bool need = inwin && x >= 0 && x < W;
-        uint64_t v = need ? ld_relaxed(a.edge + x) : 0;
-        bool ok = inwin && (!need || v != kSentinel);
-        uint64_t m = __ballot(ok);
-        int n = m == ~0ull ? 64 : (int)__builtin_ctzll(~m);  // ready prefix length
+        if (lds_ld(&hdr->abort)) return;
+        // ring capacity: the edges of finished blocks are no longer read
+        const int cap = lds_ld(&hdr->computed) * kBlk + kERing;
+        const int t = t0 + lane;
+        const int x = edge_x<KIND>(t, cols);
+        const bool inwin = t < cols && t < cap;
+        const bool need = ein && inwin && x >= 0 && x < W;
+        uint64_t v = need ? ld_relaxed(ein + x) : 0;
+        const bool ok = inwin && (!need || v != kSentinel);
+        const uint64_t m = __ballot(ok);
+        const int n = m == ~0ull ? 64 : (int)__builtin_ctzll(~m);  // ready prefix length
         if (lane < n) {
             ering[t % kERing] = as_double(v);
-            if (need) st_relaxed(a.edge + x, kSentinel);
+            if (need) st_relaxed((double*)ein + x, kSentinel);
         }
         if (n > 0) {
             t0 += n;
-            lds_rel(&hdr->edge_ready, t0);
+            lds_st(&hdr->edge_ready, t0);
             spins = 0;
         } else {
             if (++spins > kLdsSpin) {
-                lds_rel(&hdr->abort, 1);
+                lds_st(&hdr->abort, 1);
                 atomicOr(a.timeout, 1);
                 return;
             }
@@ -243,10 +287,244 @@ IFL_DEV void poller_wave(const StreamArgs& a, double* ering, RingHdr* hdr, int l
     }
 }
 
-template <int KIND, bool PM, bool VM, int MODE, int NA>
+// Wave 0. Operand registers are double-buffered by half block (4 steps):
+// a half first issues the LDS reads of the next half, then runs its steps.
+template <int KIND, int MODE, int NG>
+IFL_DEV void compute_wave(const StreamArgs& a, char* ring, RingHdr* hdr, const double* ering, int j, int lane) {
+    typedef SweepShape<KIND, MODE, NG> S;
+    constexpr int NC = S::NC;
+    const int cols = a.sk.cols, nblk = cols / kBlk, nhalf = 2 * nblk;
+    const const_u64* fm = strip_masks(a, j);
+    const double tau = a.tau, sigma = a.sigma;
+    long long* tr = a.trace ? a.trace + (long)j * nblk * kTraceStamps : nullptr;
+    auto stamp = [&](int b, int i) {
+        if (tr && lane == 0) tr[b * kTraceStamps + i] = __builtin_amdgcn_s_memrealtime();
+    };
+
+    // Readiness is cached: the counters are re-read only when the cached
+    // limit does not cover the request. Loader k has published blocks
+    // k, k + kLoaders, ..., ready[k] - 1, so every block below the smallest
+    // "next block" over the loaders is in the ring.
+    int block_limit = 0, edge_limit = 0;
+    auto ready_block = [&](int b) -> bool {
+        if (b < block_limit) return true;
+        long spins = 0;
+        while (true) {
+            int lim = 1 << 30;
+#pragma unroll
+            for (int k = 0; k < kLoaders; k++) {
+                const int r = lds_ld(&hdr->ready[k]);
+                lim = min(lim, r == 0 ? k : r - 1 + kLoaders);
+            }
+            block_limit = lim;
+            if (b < block_limit) return true;
+            if (lds_ld(&hdr->abort)) return false;
+            if (++spins > kLdsSpin) {
+                lds_st(&hdr->abort, 1);
+                atomicOr(a.timeout, 1);
+                return false;
+            }
+            __builtin_amdgcn_s_sleep(1);
+        }
+    };
+    auto ready_edges = [&](int tend) -> bool {
+        if (tend <= edge_limit) return true;
+        if (!lds_wait_gt(&hdr->edge_ready, tend - 1, hdr, a.timeout)) return false;
+        edge_limit = lds_ld(&hdr->edge_ready);
+        return true;
+    };
+
+    double XA[NC][kHalf], XB[NC][kHalf], EA[kHalf], EB[kHalf];
+    uint64_t MA[kHalf], MB[kHalf];
+    // operands of step k of half h
+    auto load_step = [&](const char* slot, int h, int k, double (&X)[NC][kHalf], double (&E)[kHalf], uint64_t (&M)[kHalf]) {
+        const int e = step_in_block<KIND>((h & 1) * kHalf + k) * kStrip + lane;
+#pragma unroll
+        for (int ai = 0; ai < NC; ai++) X[ai][k] = ((const double*)(slot + ai * 4096))[e];
+        E[k] = ering[(h * kHalf + k) % kERing];
+        M[k] = fm[step_col<KIND>(h * kHalf + k, cols)];
+    };
+    double Pw = 0.0, Pn = 0.0, zs = 0.0;
+    // half h from (X, E, M); while at it, the operands of half h+1 (in nslot) into (Y, F, N)
+    auto run_half = [&](const char* slot, int h, const double (&X)[NC][kHalf], const double (&E)[kHalf],
+                        const uint64_t (&M)[kHalf], const char* nslot, double (&Y)[NC][kHalf], double (&F)[kHalf],
+                        uint64_t (&N)[kHalf]) {
+        double* out = (double*)(slot + S::NS * 4096);
+        // Everything this half uses arrived during the previous half. Drain
+        // LDS and scalar loads now, before the next half's reads go out:
+        // scalar loads return out of order, so a wait for the masks placed
+        // after those reads would wait for all of them.
+        __builtin_amdgcn_s_waitcnt(0xC07F);  // lgkmcnt(0)
+        if (nslot) {
+#pragma unroll
+            for (int k = 0; k < kHalf; k++) load_step(nslot, h + 1, k, Y, F, N);
+        }
+        __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+        for (int k = 0; k < kHalf; k++) {
+            const int e = step_in_block<KIND>((h & 1) * kHalf + k) * kStrip + lane;
+            const bool f = lane_on(M[k]);
+            if (KIND == SW_FWD) {
+                // applyPreconditioner forward loop 7:773-787 on the updated residual
+                const double Ps = wave_shr1(Pn, E[k]);
+                const double t1 = X[0][k] - Pw;
+                const double tt = t1 - Ps;
+                const double zz = tt * X[3][k];
+                const double zm = f ? zz : 0.0;
+                Pw = X[1][k] * zm;
+                Pn = X[2][k] * zm;
+                out[e] = zm;
+            } else if (KIND == SW_BWD) {
+                // applyPreconditioner backward loop 7:789-803
+                const double zn = wave_shl1(zs, E[k]);
+                const double t1 = X[0][k] - X[1][k] * zs;
+                const double tt = t1 - X[2][k] * zn;
+                const double zz = tt * X[3][k];
+                zs = f ? zz : 0.0;
+                out[e] = zz;
+            } else {
+                // buildPreconditioner 7:744-767
+                const double Ns = wave_shr1(Pn, E[k]);
+                const double ad = X[0][k];
+                double ev = ad;
+                ev = ev - Pw;
+                ev = ev - Ns;
+                if (ev < sigma * ad) ev = ad;
+                const double pcv = 1.0 / sqrt(ev);
+                const double px = X[1][k] * pcv;
+                const double py = X[2][k] * pcv;
+                const double eo = px * px + tau * px * py;
+                const double no = py * py + tau * px * py;
+                Pw = f ? eo : 0.0;
+                Pn = f ? no : 0.0;
+                out[e] = pcv;
+            }
+        }
+    };
+
+    if (!ready_block(0) || !ready_edges(kHalf)) return;
+    const char* slot = ring;  // block 0 is in slot 0
+#pragma unroll
+    for (int k = 0; k < kHalf; k++) load_step(slot, 0, k, XA, EA, MA);
+    int sidx = 0;
+    for (int b = 0; b < nblk; b++) {
+        stamp(b, 0);
+        // second half of this block: same slot
+        if (!ready_edges(b * kBlk + kBlk)) return;
+        stamp(b, 1);
+        run_half(slot, 2 * b, XA, EA, MA, slot, XB, EB, MB);
+        stamp(b, 2);
+        // first half of the next block
+        const int nidx = sidx + 1 == a.nb ? 0 : sidx + 1;
+        const char* nslot = ring + (long)nidx * a.slot_bytes;
+        const bool more = b + 1 < nblk;
+        if (more && (!ready_block(b + 1) || !ready_edges(b * kBlk + kBlk + kHalf))) return;
+        stamp(b, 3);
+        run_half(slot, 2 * b + 1, XB, EB, MB, more ? nslot : nullptr, XA, EA, MA);
+        lds_st(&hdr->computed, b + 1);
+        stamp(b, 4);
+        slot = nslot;
+        sidx = nidx;
+    }
+    (void)nhalf;
+}
+
+// Wave 2: drains finished blocks.
+template <int KIND, int MODE, int LV, int NG>
+IFL_DEV void writer_wave(const StreamArgs& a, char* ring, RingHdr* hdr, int j, int lane, double* eout) {
+    typedef SweepShape<KIND, MODE, NG> S;
+    constexpr bool VM = LV >= 6;   // vector operations masked to FLUID cells
+    constexpr bool AUX = LV == 4;  // levels 4-5: the dot reads the reference's _z at non-FLUID cells
+    const int W = a.sk.W, H = a.sk.H, cols = a.sk.cols, nblk = cols / kBlk;
+    const const_u64* fm = strip_masks(a, j);
+    const long sbase = (long)j * a.sk.strip_sz;
+    const bool row_ok = j * kStrip + lane < H;
+    const double tau = a.tau;
+    double rmax = 0.0, acc = 0.0;
+    for (int b = 0; b < nblk; b++) {
+        if (!lds_wait_gt(&hdr->computed, b, hdr, a.timeout)) return;
+        char* slot = ring + (long)(b % a.nb) * a.slot_bytes;
+        double* in = (double*)slot;
+        const double* out = (const double*)(slot + S::NS * 4096);
+        const int col0 = block_col0<KIND>(b, cols);
+        const long gblk = sbase + (long)col0 * kStrip;  // first element of the block in HBM
+#pragma unroll
+        for (int k = 0; k < kBlk; k++) {
+            const int t = b * kBlk + k;
+            const int c = step_col<KIND>(t, cols);
+            const int e = step_in_block<KIND>(k) * kStrip + lane;
+            const int x = c - lane;
+            const bool valid = row_ok && x >= 0 && x < W;
+            const bool f = lane_on(fm[c]);
+            if (KIND == SW_FWD) {
+                // infinityNorm(_r) 7:871 (masked in 6+) of the residual the sweep used
+                if (VM ? f : valid) rmax = smax(rmax, fabs(in[e]));
+            } else if (KIND == SW_BWD) {
+                // store _z at FLUID cells; dotProduct(_z, _r) 7:880 (unmasked in 3-5)
+                const double zz = out[e];
+                if (f) a.out[0][gblk + e] = zz;
+                const double zdot = f ? zz : (AUX ? in[5 * 512 + e] : 0.0);
+                if (VM ? f : valid) acc += zdot * in[4 * 512 + e];
+            } else {
+                // the factor at FLUID cells; cx = aPlusX*precon, cy likewise, +0 elsewhere
+                const double pcv = out[e];
+                if (f) a.out[0][gblk + e] = pcv;
+                const double px = in[512 + e] * pcv, py = in[1024 + e] * pcv;
+                in[512 + e] = f ? px : 0.0;
+                in[1024 + e] = f ? py : 0.0;
+            }
+        }
+        // the boundary row for the neighbouring strip, one step per lane;
+        // the same operations on the same operands as wave 0 performed
+        if (eout && lane < kBlk) {
+            const int t = b * kBlk + lane;
+            const int c = step_col<KIND>(t, cols);
+            const int e = step_in_block<KIND>(lane) * kStrip + (KIND == SW_BWD ? 0 : kStrip - 1);
+            const int x = KIND == SW_BWD ? c : c - (kStrip - 1);
+            if (x >= 0 && x < W) {
+                double v;
+                if (KIND == SW_FWD) {
+                    v = in[2 * 512 + e] * out[e];  // Pn = cy * zm of lane 63
+                } else if (KIND == SW_BWD) {
+                    v = (fm[c] & 1ull) ? out[e] : 0.0;  // zs of lane 0
+                } else {
+                    const double px = in[512 + e], py = in[1024 + e];  // +0 off FLUID
+                    v = ((fm[c] >> 63) & 1ull) ? py * py + tau * px * py : 0.0;
+                }
+                st_relaxed(eout + x, as_bits(v));
+            }
+        }
+        // whole-block stores of the unmasked outputs, 1 KB per instruction
+        auto store_block = [&](double* dst, const double* src) {
+#pragma unroll
+            for (int m = 0; m < 4; m++) {
+                const vec2d v = ((const vec2d*)src)[m * 64 + lane];
+                *(vec2d*)(dst + gblk + m * 128 + lane * 2) = v;
+            }
+        };
+        if (KIND == SW_FWD) {
+            store_block(a.out[0], out);
+            if (MODE == 1) store_block(a.out[1], in);
+        } else if (KIND == SW_PRECON) {
+            store_block(a.out[1], in + 512);
+            store_block(a.out[2], in + 1024);
+        }
+        if (a.trace && lane == 0) a.trace[((long)j * nblk + b) * kTraceStamps + 5] = __builtin_amdgcn_s_memrealtime();
+        lds_st(&hdr->consumed, b + 1);
+    }
+    if (KIND == SW_FWD) {
+        rmax = wave_max(rmax);
+        if (lane == 0) atomicMax((unsigned long long*)&a.st->max_err, (unsigned long long)as_bits(rmax));
+    } else if (KIND == SW_BWD) {
+        acc = wave_sum(acc);
+        if (lane == 0) a.partial[j] = acc;
+    }
+}
+
+template <int KIND, int MODE, int LV, int NG>
 __global__ __launch_bounds__(64 * kWaves) void stream_sweep_kernel(StreamArgs a) {
     extern __shared__ __align__(16) char smem[];
-    const int j = strip_of_block(blockIdx.x, a.sk.nstrips);
+    const int j = strip_of_block(blockIdx.x, a.strip0, a.nlaunch);
     if (j < 0) return;
     PcgState* st = a.st;
     if (KIND != SW_PRECON && ((MODE == 1 && st->done) || st->timeout)) return;
@@ -257,160 +535,22 @@ __global__ __launch_bounds__(64 * kWaves) void stream_sweep_kernel(StreamArgs a)
     if (threadIdx.x < (int)(sizeof(RingHdr) / sizeof(int))) ((int*)hdr)[threadIdx.x] = 0;
     __syncthreads();
 
-    const int W = a.sk.W, H = a.sk.H, cols = a.sk.cols;
-    const int nblk = cols / kBlk;
+    const int W = a.sk.W, nblk = a.sk.cols / kBlk;
     const long sbase = (long)j * a.sk.strip_sz;
     // edge source: the strip below (fwd, precon) or above (bwd)
     const bool has_in = KIND == SW_BWD ? j + 1 < a.sk.nstrips : j > 0;
     const bool has_out = KIND == SW_BWD ? j > 0 : j + 1 < a.sk.nstrips;
-    double* ein = has_in ? a.edge + (long)(KIND == SW_BWD ? j + 1 : j) * W : nullptr;
+    const double* ein = has_in ? a.edge + (long)(KIND == SW_BWD ? j + 1 : j) * W : nullptr;
     double* eout = has_out ? a.edge + (long)(KIND == SW_BWD ? j : j + 1) * W : nullptr;
 
-    if (wave >= 2) {
-        loader_wave<KIND, NA>(a, ring, hdr, wave - 2, lane, sbase, nblk);
-        return;
-    }
-    if (wave == 1) {
-        if (ein) {
-            StreamArgs b = a;
-            b.edge = ein;
-            poller_wave<KIND>(b, ering, hdr, lane);
-        }
-        return;
-    }
-
-    // ---- wave 0: the recurrence ----
-    const bool row_ok = j * kStrip + lane < H;
-    const long gbase = sbase + lane;
-    constexpr int na = NA;
-    const double alpha_neg = (KIND == SW_FWD && MODE == 1) ? -st->alpha : 0.0;
-    const double tau = a.tau, sigma = a.sigma;
-    // 4-step rolling window of operands read from the ring (3 steps ahead)
-    double w0[4], w1[4], w2[4], w3[4], w4[4], w5[4], wev[4];
-    uint8_t wf[4];
-    auto rd = [&](int t, int wi) {
-        const int b = t >> 3, k = t & 7;
-        const char* slot = ring + (long)(b % a.nb) * a.slot_bytes;
-        const double* s0 = (const double*)slot;
-        const int e = (KIND == SW_BWD ? (kBlk - 1 - k) : k) * kStrip + lane;
-        w0[wi] = s0[e];
-        w1[wi] = s0[512 + e];
-        w2[wi] = s0[1024 + e];
-        if (na > 3) w3[wi] = s0[1536 + e];
-        if (na > 4) w4[wi] = s0[2048 + e];
-        if (na > 5) w5[wi] = s0[2560 + e];
-        wf[wi] = ((const uint8_t*)(slot + na * 4096))[e];
-        wev[wi] = ein ? ering[t % kERing] : 0.0;
-    };
-    auto ready = [&](int b) -> bool {
-        if (!lds_wait_gt(&hdr->ready[b % kLoaders], b, hdr, a.timeout)) return false;
-        if (ein && !lds_wait_gt(&hdr->edge_ready, b * kBlk + kBlk - 1, hdr, a.timeout)) return false;
-        return true;
-    };
-
-    double Pw = 0.0, Pn = 0.0, rmax = 0.0;  // fwd / precon carries (precon: Ew, En)
-    double zs = 0.0, acc = 0.0;             // bwd carries
-    if (!ready(0)) return;
-    rd(0, 0);
-    rd(1, 1);
-    rd(2, 2);
-    for (int b = 0; b < nblk; b++) {
-        if (a.trace && lane == 0) a.trace[((long)blockIdx.x * nblk + b) * 3 + 0] = __builtin_amdgcn_s_memrealtime();
-        if (b + 1 < nblk && !ready(b + 1)) return;
-        if (a.trace && lane == 0) a.trace[((long)blockIdx.x * nblk + b) * 3 + 1] = __builtin_amdgcn_s_memrealtime();
-        const int T = b * kBlk;
-#pragma unroll
-        for (int k = 0; k < kBlk; k++) {
-            const int t = T + k;
-            if (t + 3 < cols) rd(t + 3, (k + 3) & 3);
-            const int wi = k & 3;
-            const bool fl = wf[wi] != 0;
-            if (KIND == SW_FWD) {
-                // applyPreconditioner forward loop 7:773-787 (+ scaledAdd / infinityNorm in MODE 1)
-                const int x = t - lane;
-                const bool valid = row_ok && x >= 0 && x < W;
-                const long p = gbase + (long)t * kStrip;
-                const double Ps = wave_shr1(Pn, wev[wi]);
-                double r = w0[wi];
-                const bool vec_on = valid && (!VM || fl);
-                if (MODE == 1) {
-                    double rn = r + w4[wi] * alpha_neg;
-                    if (vec_on) {
-                        r = rn;
-                        a.out1[p] = rn;
-                    }
-                }
-                if (vec_on) rmax = smax(rmax, fabs(r));
-                const double t1 = r - Pw;
-                const double tt = t1 - Ps;
-                const double zz = tt * w3[wi];
-                double pw = w1[wi] * zz;
-                double pn = w2[wi] * zz;
-                if (PM) {
-                    pw = fl ? pw : 0.0;
-                    pn = fl ? pn : 0.0;
-                }
-                Pw = pw;
-                Pn = pn;
-                if (valid && (!PM || fl)) a.out0[p] = zz;
-                if (eout && lane == kStrip - 1 && valid) st_relaxed(eout + x, as_bits(pn));
-            } else if (KIND == SW_BWD) {
-                // applyPreconditioner backward loop 7:789-803 + dotProduct(_z, _r)
-                const int c = cols - 1 - t;
-                const int x = c - lane;
-                const bool valid = row_ok && x >= 0 && x < W;
-                const long p = gbase + (long)c * kStrip;
-                const double zn = wave_shl1(zs, wev[wi]);
-                const double t1 = w0[wi] - w1[wi] * zs;
-                const double tt = t1 - w2[wi] * zn;
-                const double zz = tt * w3[wi];
-                zs = PM ? (fl ? zz : 0.0) : zz;
-                if (valid) {
-                    double zdot = zz;
-                    if (!PM || fl)
-                        a.out0[p] = zz;
-                    else if (PM && !VM)
-                        zdot = w5[wi];  // the reference's _z there: A s (MODE 1) or the stale value (MODE 0)
-                    if (!VM || fl) acc += zdot * w4[wi];
-                }
-                if (eout && lane == 0 && valid) st_relaxed(eout + x, as_bits(zs));
-            } else {
-                // buildPreconditioner 7:744-767
-                const int x = t - lane;
-                const bool valid = row_ok && x >= 0 && x < W;
-                const double Ns = wave_shr1(Pn, wev[wi]);
-                const double ad = w0[wi];
-                double e = ad;
-                e = e - Pw;
-                e = e - Ns;
-                if (e < sigma * ad) e = ad;
-                const double pcv = 1.0 / sqrt(e);
-                const double px = w1[wi] * pcv;
-                const double py = w2[wi] * pcv;
-                const double eo = px * px + tau * px * py;
-                const double no = py * py + tau * px * py;
-                const bool on = valid && fl;
-                Pw = on ? eo : 0.0;
-                Pn = on ? no : 0.0;
-                if (on) {
-                    const long p = gbase + (long)t * kStrip;
-                    a.out0[p] = pcv;
-                    a.out1[p] = px;
-                    a.out2[p] = py;
-                }
-                if (eout && lane == kStrip - 1 && valid) st_relaxed(eout + x, as_bits(Pn));
-            }
-        }
-        // every read of slot b has been consumed by now: hand it back
-        __hip_atomic_store(&hdr->consumed, b + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-    }
-    if (KIND == SW_FWD) {
-        rmax = wave_max(rmax);
-        if (lane == 0) atomicMax((unsigned long long*)&st->max_err, (unsigned long long)as_bits(rmax));
-    } else if (KIND == SW_BWD) {
-        acc = wave_sum(acc);
-        if (lane == 0) a.partial[j] = acc;
-    }
+    if (wave >= 3)
+        loader_wave<KIND, MODE, LV, NG>(a, ring, hdr, wave - 3, lane, sbase, nblk, j);
+    else if (wave == 2)
+        writer_wave<KIND, MODE, LV, NG>(a, ring, hdr, j, lane, eout);
+    else if (wave == 1)
+        poller_wave<KIND>(a, ein, ering, hdr, lane);
+    else
+        compute_wave<KIND, MODE, NG>(a, ring, hdr, ering, j, lane);
 }
 
 // ---------------------------------------------------------------- vector phases
@@ -660,7 +800,7 @@ static StreamArgs stream_args(Solver* s, int kind) {
     memset(&a, 0, sizeof a);
     a.trace = nullptr;
     a.sk = s->sk;
-    a.flag = s->dflag;
+    a.fmask = s->fmask;
     a.edge = s->edge + (long)kind * s->sk.nstrips * s->W;
     a.partial = s->partial;
     a.st = s->st;
@@ -670,14 +810,13 @@ static StreamArgs stream_args(Solver* s, int kind) {
     return a;
 }
 
-template <int KIND, bool PM, bool VM, int MODE, int NA>
+template <int KIND, int MODE, int LV, int NA>
 static int launch_stream(Solver* s, StreamArgs a) {
-    a.na = NA;
-    a.slot_bytes = NA * 4096 + 512;
+    a.slot_bytes = SweepShape<KIND, MODE, NA>::SLOT;
     size_t fixed = sizeof(RingHdr) + kERing * sizeof(double);
     int nb = (int)((s->lds_bytes - fixed) / a.slot_bytes);
     if (nb > 12) nb = 12;
-    if (nb < 2) {
+    if (nb < 3) {
         s->err = "not enough LDS for the sweep ring";
         return IFL_ERR_UNSUPPORTED;
     }
@@ -685,72 +824,69 @@ static int launch_stream(Solver* s, StreamArgs a) {
     size_t shmem = fixed + (size_t)nb * a.slot_bytes;
     static bool attr_set = false;
     if (!attr_set) {
-        IFL_HIP(hipFuncSetAttribute((const void*)stream_sweep_kernel<KIND, PM, VM, MODE, NA>,
+        IFL_HIP(hipFuncSetAttribute((const void*)stream_sweep_kernel<KIND, MODE, LV, NA>,
                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)shmem));
         attr_set = true;
     }
-    stream_sweep_kernel<KIND, PM, VM, MODE, NA><<<sweep_blocks(s->sk.nstrips), 64 * kWaves, shmem, s->stream>>>(a);
-    IFL_LAUNCH_CHECK();
+    // A strip waits for its upstream neighbour, so every strip of a launch
+    // must be resident at once (one workgroup per CU): taller grids run as
+    // several launches in sweep order, the edges crossing through memory.
+    const int ns = s->sk.nstrips, chunk = std::max(8, s->num_cus / 8 * 8);
+    for (int done = 0; done < ns; done += chunk) {
+        a.nlaunch = std::min(chunk, ns - done);
+        a.strip0 = KIND == SW_BWD ? ns - done - a.nlaunch : done;
+        stream_sweep_kernel<KIND, MODE, LV, NA><<<sweep_blocks(a.nlaunch), 64 * kWaves, shmem, s->stream>>>(a);
+        IFL_LAUNCH_CHECK();
+    }
     return IFL_OK;
 }
 
 template <int MODE>
-static int launch_fwd(Solver* s) {
+static int launch_fwd(Solver* s, long long* trace = nullptr) {
     StreamArgs a = stream_args(s, SW_FWD);
     a.in[0] = s->r; a.in[1] = s->cx; a.in[2] = s->cy; a.in[3] = s->precon; a.in[4] = s->qv;
-    a.out0 = s->yv;
-    a.out1 = s->r;
+    a.out[0] = s->yv;
+    a.out[1] = s->r;
+    a.trace = trace;
     constexpr int NA = MODE == 1 ? 5 : 4;
-    if (s->level == 3) return launch_stream<SW_FWD, false, false, MODE, NA>(s, a);
-    if (s->level <= 5) return launch_stream<SW_FWD, true, false, MODE, NA>(s, a);
-    return launch_stream<SW_FWD, true, true, MODE, NA>(s, a);
+    if (s->level == 3) return launch_stream<SW_FWD, MODE, 3, NA>(s, a);
+    if (s->level <= 5) return launch_stream<SW_FWD, MODE, 4, NA>(s, a);
+    return launch_stream<SW_FWD, MODE, 6, NA>(s, a);
 }
 
 template <int MODE>
-static int launch_bwd(Solver* s) {
+static int launch_bwd(Solver* s, long long* trace = nullptr) {
     StreamArgs a = stream_args(s, SW_BWD);
     a.in[0] = s->yv; a.in[1] = s->cx; a.in[2] = s->cy; a.in[3] = s->precon; a.in[4] = s->r;
     a.in[5] = MODE == 1 ? s->qv : s->z;  // the reference's _z at non-FLUID cells (levels 4-5 dot)
-    a.out0 = s->z;
-    if (s->level == 3) return launch_stream<SW_BWD, false, false, MODE, 5>(s, a);
-    if (s->level <= 5) return launch_stream<SW_BWD, true, false, MODE, 6>(s, a);
-    return launch_stream<SW_BWD, true, true, MODE, 5>(s, a);
+    a.out[0] = s->z;
+    a.trace = trace;
+    if (s->level == 3) return launch_stream<SW_BWD, MODE, 3, 5>(s, a);
+    if (s->level <= 5) return launch_stream<SW_BWD, MODE, 4, 6>(s, a);
+    return launch_stream<SW_BWD, MODE, 6, 5>(s, a);
 }
 
 static long skew_n(Solver* s) { return (long)s->sk.nstrips * s->sk.strip_sz; }
 
-int k_build_precon(Solver* s) {
+static int launch_precon(Solver* s, long long* trace) {
     StreamArgs a = stream_args(s, SW_PRECON);
     a.in[0] = s->adiag; a.in[1] = s->aplusx; a.in[2] = s->aplusy;
-    a.out0 = s->precon;
-    a.out1 = s->cx;
-    a.out2 = s->cy;
-    return launch_stream<SW_PRECON, true, false, 0, 3>(s, a);
+    a.out[0] = s->precon;
+    a.out[1] = s->cx;
+    a.out[2] = s->cy;
+    a.trace = trace;
+    return launch_stream<SW_PRECON, 0, 3, 3>(s, a);
 }
 
-// Diagnostics: one sweep of `kind` on the current state with the compute
-// wave's waits and the loaders' publications time-stamped (100 MHz clock).
+int k_build_precon(Solver* s) { return launch_precon(s, nullptr); }
+
+// Diagnostics: one sweep of `kind` on the current state, time-stamped per
+// block (100 MHz clock): wave 0 starts the block, its operands for the
+// second half are ready, the writer has drained it.
 int k_trace_sweep(Solver* s, int kind, long long* dev_trace) {
-    StreamArgs a;
-    if (kind == SW_PRECON) {
-        a = stream_args(s, SW_PRECON);
-        a.in[0] = s->adiag; a.in[1] = s->aplusx; a.in[2] = s->aplusy;
-        a.out0 = s->precon; a.out1 = s->cx; a.out2 = s->cy;
-        a.trace = dev_trace;
-        return launch_stream<SW_PRECON, true, false, 0, 3>(s, a);
-    }
-    if (kind == SW_FWD) {
-        a = stream_args(s, SW_FWD);
-        a.in[0] = s->r; a.in[1] = s->cx; a.in[2] = s->cy; a.in[3] = s->precon; a.in[4] = s->qv;
-        a.out0 = s->yv; a.out1 = s->r;
-        a.trace = dev_trace;
-        return s->level >= 6 ? launch_stream<SW_FWD, true, true, 0, 4>(s, a) : launch_stream<SW_FWD, true, false, 0, 4>(s, a);
-    }
-    a = stream_args(s, SW_BWD);
-    a.in[0] = s->yv; a.in[1] = s->cx; a.in[2] = s->cy; a.in[3] = s->precon; a.in[4] = s->r; a.in[5] = s->z;
-    a.out0 = s->z;
-    a.trace = dev_trace;
-    return s->level >= 6 ? launch_stream<SW_BWD, true, true, 0, 5>(s, a) : launch_stream<SW_BWD, true, false, 0, 6>(s, a);
+    if (kind == SW_PRECON) return launch_precon(s, dev_trace);
+    if (kind == SW_FWD) return launch_fwd<0>(s, dev_trace);
+    return launch_bwd<0>(s, dev_trace);
 }
 
 int k_apply_precon(Solver* s) {
