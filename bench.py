@@ -23,6 +23,17 @@ PEAK_HBM_GBS = 8000.0
 # forward sweep 40, backward sweep 48, s update 24) plus the 5 one-byte cell
 # flag reads of the FLUID-masked solvers 6-8.
 PCG_BYTES_PER_CELL_ITER = {3: 200.0, 4: 200.0, 5: 200.0, 6: 205.0, 7: 205.0, 8: 205.0}
+# Algorithmic bytes per cell of one launch of the per-iteration kernels
+# (solver 7): each array the kernel must read or write, once.
+KERNEL_BYTES_PER_CELL = {
+    # r, cx, cy, precon, q read; r, y written (the 64-lane FLUID masks add 1/8 B)
+    "sweep_fwd": 56.0,
+    # y, cx, cy, precon, r read; z written
+    "sweep_bwd": 48.0,
+    # adiag, aplusx, aplusy, s read; q written
+    "matvec": 40.0,
+}
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "traffic.json")
 
 
 def parse():
@@ -34,7 +45,20 @@ def parse():
     ap.add_argument("--form", choices=["fp64", "mixed"], default="fp64")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-iters", type=int, default=30)
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="only run a plain 1 GiB device copy (scale for rocprofv3 byte counters) and exit")
     return ap.parse_args()
+
+
+def calibrate_copy():
+    import torch
+    n = 1 << 27  # doubles: 1 GiB
+    src = torch.ones(n, dtype=torch.float64, device="cuda")
+    dst = torch.empty_like(src)
+    for _ in range(3):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    print(json.dumps({"calibration": "device copy", "bytes_read": n * 8, "bytes_written": n * 8, "launches": 3}))
 
 
 def cpu_baseline(scene, n, mean_iters):
@@ -88,6 +112,18 @@ def cpu_baseline(scene, n, mean_iters):
 cpu_iters_global = 12
 
 
+def measured_traffic(kernel):
+    """HBM bytes per launch of `kernel` from the rocprofv3 --pmc FETCH_SIZE and
+    WRITE_SIZE runs recorded in profiles/traffic.json (each scaled by the same
+    counter's reading for a device copy of known size), or None."""
+    try:
+        with open(TRAFFIC_FILE) as f:
+            t = json.load(f)
+        return t["kernels"][kernel]["hbm_bytes_per_launch"]
+    except (OSError, KeyError, ValueError):
+        return None
+
+
 def main():
     global cpu_iters_global
     args = parse()
@@ -96,6 +132,9 @@ def main():
                                    "(the multi-GPU strip decomposition is a design note only)"}))
         sys.exit(2)
     cpu_iters_global = args.cpu_iters
+    if args.calibrate_copy:
+        calibrate_copy()
+        return
     import ifl_fluids as F
     import ifl_scenes as S
 
@@ -134,6 +173,17 @@ def main():
     solver.synchronize()
     wall = time.perf_counter() - t0
     prof = solver.profile_read(F.KERNEL_PCG)
+    kern = {}
+    for name, kid in (("sweep_fwd", F.KERNEL_SWEEP_FWD), ("sweep_bwd", F.KERNEL_SWEEP_BWD),
+                      ("matvec", F.KERNEL_MATVEC)):
+        p = solver.profile_read(kid)
+        if p["launches"] == 0:
+            continue
+        avg_ms = p["ms"] / p["launches"]
+        bytes_launch = KERNEL_BYTES_PER_CELL[name] * n * n
+        kern[name] = {"launches": p["launches"], "total_ms": p["ms"], "avg_us": 1e3 * avg_ms,
+                      "bytes_per_launch": bytes_launch, "achieved_GBps": bytes_launch / (avg_ms * 1e-3) / 1e9}
+    dom = max(kern, key=lambda k: kern[k]["total_ms"]) if kern else None
 
     steps_per_s = args.steps / wall
     pcg_iters = prof["iterations"]
@@ -161,10 +211,14 @@ def main():
         "pcg": {"launches": prof["launches"], "ms_total": prof["ms"], "iterations": pcg_iters,
                 "ms_per_iteration": prof["ms"] / max(1, pcg_iters), "bytes_per_cell_iteration": bpc,
                 "achieved_GBps": pcg_gbs},
-        "roofline": {"bound": "hbm", "achieved": pcg_gbs, "peak": PEAK_HBM_GBS, "unit": "GB/s",
-                     "frac": pcg_gbs / PEAK_HBM_GBS, "traffic": None,
-                     "kernel": "project() (one MIC(0)-PCG solve), algorithmic bytes 205 B/cell/iteration"},
+        "kernels": kern,
     }
+    if dom:
+        k = kern[dom]
+        out["roofline"] = {"bound": "hbm", "achieved": k["achieved_GBps"], "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                           "frac": k["achieved_GBps"] / PEAK_HBM_GBS, "traffic": measured_traffic(dom),
+                           "kernel": f"{dom}: {KERNEL_BYTES_PER_CELL[dom]:.0f} B/cell x {n}x{n} cells per launch, "
+                                     f"average over {k['launches']} launches (HIP events on the solver's stream)"}
     solver.close()
     if not args.no_cpu_baseline:
         try:

@@ -197,14 +197,19 @@ int ifl_stage(ifl_solver* s, int stage, int iarg, double darg,
 
 /* ---- timing of the device work, HIP events on the solver's stream ---- */
 enum {
-    IFL_KERNEL_PCG = 0,       /* the persistent project() kernel, one launch per solve */
-    IFL_KERNEL_PRECON = 1,    /* buildPreconditioner wavefront */
-    IFL_KERNEL_ADVECT = 2,
-    IFL_KERNEL_OTHER = 3,
-    IFL_KERNEL_COUNT = 4
+    IFL_KERNEL_PCG = 0,        /* one project() solve, all of its kernels timed as one */
+    IFL_KERNEL_PRECON = 1,     /* buildPreconditioner wavefront sweep, one launch per solve */
+    IFL_KERNEL_ADVECT = 2,     /* advection of the quantities of one update() */
+    IFL_KERNEL_SWEEP_FWD = 3,  /* r -= alpha*q, max|r| and the forward MIC(0) sweep, one launch per PCG iteration */
+    IFL_KERNEL_SWEEP_BWD = 4,  /* backward MIC(0) sweep and the z.r partials, one launch per PCG iteration */
+    IFL_KERNEL_MATVEC = 5,     /* q = A s and the q.s partials, one launch per PCG iteration */
+    IFL_KERNEL_COUNT = 6
 };
 int ifl_profile_enable(ifl_solver* s, int on);
-/* launches, summed milliseconds, summed PCG iterations, summed cell-iterations */
+/* launches, summed milliseconds, summed PCG iterations (project() only) and
+ * summed cells processed (cells x launches, or x iterations for project()).
+ * Only launches that did work are counted: a speculatively queued PCG
+ * iteration after convergence is not. */
 int ifl_profile_read(const ifl_solver* s, int kernel, int64_t* launches,
                      double* total_ms, int64_t* iterations, double* cell_iterations);
 int ifl_profile_reset(ifl_solver* s);

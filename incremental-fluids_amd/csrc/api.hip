@@ -13,6 +13,56 @@ struct ifl_solver {
     Solver s;
 };
 
+namespace ifl {
+
+static int ev_take(Solver* s, hipEvent_t* e) {
+    if (!s->ev_free.empty()) {
+        *e = s->ev_free.back();
+        s->ev_free.pop_back();
+        return IFL_OK;
+    }
+    IFL_HIP(hipEventCreate(e));
+    return IFL_OK;
+}
+
+int prof_begin(Solver* s, int kernel, int index) {
+    if (!s->profiling) return IFL_OK;
+    Solver::Timed t = {kernel, index, nullptr, nullptr};
+    IFL_TRY(ev_take(s, &t.a));
+    IFL_HIP(hipEventRecord(t.a, s->stream));
+    s->timed.push_back(t);
+    return IFL_OK;
+}
+
+int prof_end(Solver* s) {
+    if (!s->profiling || s->timed.empty() || s->timed.back().b) return IFL_OK;
+    IFL_TRY(ev_take(s, &s->timed.back().b));
+    IFL_HIP(hipEventRecord(s->timed.back().b, s->stream));
+    return IFL_OK;
+}
+
+int prof_collect(Solver* s, const int* index_limit) {
+    if (s->timed.empty()) return IFL_OK;
+    IFL_HIP(hipStreamSynchronize(s->stream));
+    for (auto& t : s->timed) {
+        const bool counted = t.b && (t.index < 0 || !index_limit || t.index < index_limit[t.kernel]);
+        if (counted) {
+            float ms = 0;
+            IFL_HIP(hipEventElapsedTime(&ms, t.a, t.b));
+            Profile& pr = s->prof[t.kernel];
+            pr.launches += 1;
+            pr.ms += ms;
+            pr.cell_iterations += (double)s->W * s->H;
+        }
+        s->ev_free.push_back(t.a);
+        if (t.b) s->ev_free.push_back(t.b);
+    }
+    s->timed.clear();
+    return IFL_OK;
+}
+
+}  // namespace ifl
+
 namespace {
 
 thread_local std::string g_err;
@@ -119,6 +169,13 @@ void free_all(Solver* s) {
     if (s->st_host) (void)hipHostFree(s->st_host);
     if (s->ev0) (void)hipEventDestroy(s->ev0);
     if (s->ev1) (void)hipEventDestroy(s->ev1);
+    for (auto& t : s->timed) {
+        if (t.a) (void)hipEventDestroy(t.a);
+        if (t.b) (void)hipEventDestroy(t.b);
+    }
+    s->timed.clear();
+    for (hipEvent_t e : s->ev_free) (void)hipEventDestroy(e);
+    s->ev_free.clear();
     if (s->stream) (void)hipStreamDestroy(s->stream);
 }
 
@@ -280,6 +337,7 @@ int update_grid(Solver* s, double dt, ifl_step_stats* stats) {
         flip_q(s, IFL_Q_V);
     }
     IFL_HIP(hipStreamSynchronize(s->stream));
+    IFL_TRY(prof_collect(s, nullptr));
     if (stats) {
         stats->heat = heat;
         stats->pressure = press;

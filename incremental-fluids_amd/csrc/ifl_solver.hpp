@@ -6,6 +6,7 @@
 
 #include <cstring>
 #include <string>
+#include <vector>
 
 #include "ifl_common.hpp"
 
@@ -88,6 +89,12 @@ struct Solver {
     bool profiling = false;
     Profile prof[IFL_KERNEL_COUNT];
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    struct Timed {
+        int kernel, index;  // index: PCG iteration of the launch, or -1 (always counted)
+        hipEvent_t a, b;
+    };
+    std::vector<hipEvent_t> ev_free;  // event pool for per-launch timing
+    std::vector<Timed> timed;         // recorded, not yet collected
 
     int border_solid = 0;
     size_t lds_bytes = 65536;  // dynamic LDS available to one workgroup
@@ -143,6 +150,13 @@ int k_matvec(Solver* s);             // z = A s, the stage form
 int k_dot(Solver* s, int which, double* out);  // 0: z.r  1: z.s
 int k_inf_norm(Solver* s, double* out);
 int k_project(Solver* s, int limit, ifl_solve_stats* stats);
+// per-launch timing (when profiling): events on the solver's stream around
+// a launch; prof_collect() adds the elapsed times of launches whose index is
+// below the limit given for their kernel, after the stream has completed.
+int prof_begin(Solver* s, int kernel, int index);
+int prof_end(Solver* s);
+int prof_collect(Solver* s, const int* index_limit);
+
 // sweep diagnostics: time stamps per strip and 8-step block
 constexpr int kTraceStamps = 6;
 int k_trace_sweep(Solver* s, int kind, long long* dev_trace);

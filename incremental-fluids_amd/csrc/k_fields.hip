@@ -251,9 +251,10 @@ int k_advect(Solver* s, int qmask, double dt) {
     a.level = s->level;
     a.bodies = s->bodies_d;
     dim3 grid(grid_blocks(maxn, 256), a.nq);
+    IFL_TRY(prof_begin(s, IFL_KERNEL_ADVECT, -1));
     advect_kernel<<<grid, 256, 0, s->stream>>>(a);
     IFL_LAUNCH_CHECK();
-    return IFL_OK;
+    return prof_end(s);
 }
 
 // ------------------------------------------------------------ addInflow 7:456-473

@@ -84,14 +84,17 @@ constexpr long kLdsSpin = 1L << 22;
 
 enum { SW_FWD = 0, SW_BWD = 1, SW_PRECON = 2 };
 
-struct RingHdr {
-    int consumed;          // blocks whose slot is free again (writer)
-    int computed;          // blocks wave 0 has finished (wave 0)
+// The first eight counters are what wave 0 polls; they are read together
+// (two 16-byte LDS reads, one wait).
+struct __align__(16) RingHdr {
     int ready[kLoaders];   // per loader: 1 + last block published
     int edge_ready;        // steps whose incoming edge value is in the edge ring
     int abort;
+    int consumed;          // blocks whose slot is free again (writer)
+    int computed;          // blocks wave 0 has finished (wave 0)
     int pad[7];
 };
+static_assert(sizeof(RingHdr) == 64, "RingHdr layout");
 
 struct StreamArgs {
     Skew sk;
@@ -124,6 +127,23 @@ IFL_DEV int lds_ld(const int* p) {
     int v;
     asm volatile("ds_read_b32 %0, %1\n\ts_waitcnt lgkmcnt(0)" : "=v"(v) : "v"(lds_off(p)) : "memory");
     return __builtin_amdgcn_readfirstlane(v);
+}
+// ready[0..4], edge_ready, abort, consumed of `h`, wave-uniform
+IFL_DEV void lds_ld8(const RingHdr* h, int (&c)[8]) {
+    typedef int v4i __attribute__((ext_vector_type(4)));
+    v4i lo, hi;
+    asm volatile("ds_read_b128 %0, %2\n\tds_read_b128 %1, %2 offset:16\n\ts_waitcnt lgkmcnt(0)"
+                 : "=v"(lo), "=v"(hi)
+                 : "v"(lds_off(h))
+                 : "memory");
+    c[0] = __builtin_amdgcn_readfirstlane(lo.x);
+    c[1] = __builtin_amdgcn_readfirstlane(lo.y);
+    c[2] = __builtin_amdgcn_readfirstlane(lo.z);
+    c[3] = __builtin_amdgcn_readfirstlane(lo.w);
+    c[4] = __builtin_amdgcn_readfirstlane(hi.x);
+    c[5] = __builtin_amdgcn_readfirstlane(hi.y);
+    c[6] = __builtin_amdgcn_readfirstlane(hi.z);
+    c[7] = __builtin_amdgcn_readfirstlane(hi.w);
 }
 IFL_DEV void lds_st(int* p, int v) {
     asm volatile("ds_write_b32 %0, %1" : : "v"(lds_off(p)), "v"(v) : "memory");
@@ -306,19 +326,21 @@ IFL_DEV void compute_wave(const StreamArgs& a, char* ring, RingHdr* hdr, const d
     // k, k + kLoaders, ..., ready[k] - 1, so every block below the smallest
     // "next block" over the loaders is in the ring.
     int block_limit = 0, edge_limit = 0;
-    auto ready_block = [&](int b) -> bool {
-        if (b < block_limit) return true;
+    // true once block b is in the ring and the edges of steps < tend are in
+    // the edge ring
+    auto ready = [&](int b, int tend) -> bool {
+        if (b < block_limit && tend <= edge_limit) return true;
         long spins = 0;
         while (true) {
+            int c[8];
+            lds_ld8(hdr, c);
             int lim = 1 << 30;
 #pragma unroll
-            for (int k = 0; k < kLoaders; k++) {
-                const int r = lds_ld(&hdr->ready[k]);
-                lim = min(lim, r == 0 ? k : r - 1 + kLoaders);
-            }
+            for (int k = 0; k < kLoaders; k++) lim = min(lim, c[k] == 0 ? k : c[k] - 1 + kLoaders);
             block_limit = lim;
-            if (b < block_limit) return true;
-            if (lds_ld(&hdr->abort)) return false;
+            edge_limit = c[kLoaders];
+            if (b < block_limit && tend <= edge_limit) return true;
+            if (c[kLoaders + 1]) return false;  // abort
             if (++spins > kLdsSpin) {
                 lds_st(&hdr->abort, 1);
                 atomicOr(a.timeout, 1);
@@ -326,12 +348,6 @@ IFL_DEV void compute_wave(const StreamArgs& a, char* ring, RingHdr* hdr, const d
             }
             __builtin_amdgcn_s_sleep(1);
         }
-    };
-    auto ready_edges = [&](int tend) -> bool {
-        if (tend <= edge_limit) return true;
-        if (!lds_wait_gt(&hdr->edge_ready, tend - 1, hdr, a.timeout)) return false;
-        edge_limit = lds_ld(&hdr->edge_ready);
-        return true;
     };
 
     double XA[NC][kHalf], XB[NC][kHalf], EA[kHalf], EB[kHalf];
@@ -402,7 +418,7 @@ IFL_DEV void compute_wave(const StreamArgs& a, char* ring, RingHdr* hdr, const d
         }
     };
 
-    if (!ready_block(0) || !ready_edges(kHalf)) return;
+    if (!ready(0, kHalf)) return;
     const char* slot = ring;  // block 0 is in slot 0
 #pragma unroll
     for (int k = 0; k < kHalf; k++) load_step(slot, 0, k, XA, EA, MA);
@@ -410,7 +426,7 @@ IFL_DEV void compute_wave(const StreamArgs& a, char* ring, RingHdr* hdr, const d
     for (int b = 0; b < nblk; b++) {
         stamp(b, 0);
         // second half of this block: same slot
-        if (!ready_edges(b * kBlk + kBlk)) return;
+        if (!ready(b, b * kBlk + kBlk)) return;
         stamp(b, 1);
         run_half(slot, 2 * b, XA, EA, MA, slot, XB, EB, MB);
         stamp(b, 2);
@@ -418,7 +434,7 @@ IFL_DEV void compute_wave(const StreamArgs& a, char* ring, RingHdr* hdr, const d
         const int nidx = sidx + 1 == a.nb ? 0 : sidx + 1;
         const char* nslot = ring + (long)nidx * a.slot_bytes;
         const bool more = b + 1 < nblk;
-        if (more && (!ready_block(b + 1) || !ready_edges(b * kBlk + kBlk + kHalf))) return;
+        if (more && !ready(b + 1, b * kBlk + kBlk + kHalf)) return;
         stamp(b, 3);
         run_half(slot, 2 * b + 1, XB, EB, MB, more ? nslot : nullptr, XA, EA, MA);
         lds_st(&hdr->computed, b + 1);
@@ -448,6 +464,35 @@ IFL_DEV void writer_wave(const StreamArgs& a, char* ring, RingHdr* hdr, int j, i
         const double* out = (const double*)(slot + S::NS * 4096);
         const int col0 = block_col0<KIND>(b, cols);
         const long gblk = sbase + (long)col0 * kStrip;  // first element of the block in HBM
+        // the block's 8 masks in one scalar load, before any of them is used
+        uint64_t M[kBlk];
+#pragma unroll
+        for (int k = 0; k < kBlk; k++) M[k] = fm[col0 + k];
+        // first the boundary row for the neighbouring strip (it waits for it),
+        // one step per lane: the same operations on the same operands as
+        // wave 0 performed
+        if (eout) {
+            // lane i < 8 takes step i of the block: its column's mask
+            const int t = b * kBlk + (lane & (kBlk - 1));
+            const int c = step_col<KIND>(t, cols);
+            uint64_t mc = M[0];
+#pragma unroll
+            for (int k = 1; k < kBlk; k++) mc = c - col0 == k ? M[k] : mc;
+            const int e = step_in_block<KIND>(lane & (kBlk - 1)) * kStrip + (KIND == SW_BWD ? 0 : kStrip - 1);
+            const int x = KIND == SW_BWD ? c : c - (kStrip - 1);
+            if (lane < kBlk && x >= 0 && x < W) {
+                double v;
+                if (KIND == SW_FWD) {
+                    v = in[2 * 512 + e] * out[e];  // Pn = cy * zm of lane 63
+                } else if (KIND == SW_BWD) {
+                    v = (mc & 1ull) ? out[e] : 0.0;  // zs of lane 0
+                } else {
+                    const double px = in[512 + e] * out[e], py = in[1024 + e] * out[e];
+                    v = ((mc >> 63) & 1ull) ? py * py + tau * px * py : 0.0;
+                }
+                st_relaxed(eout + x, as_bits(v));
+            }
+        }
 #pragma unroll
         for (int k = 0; k < kBlk; k++) {
             const int t = b * kBlk + k;
@@ -455,7 +500,7 @@ IFL_DEV void writer_wave(const StreamArgs& a, char* ring, RingHdr* hdr, int j, i
             const int e = step_in_block<KIND>(k) * kStrip + lane;
             const int x = c - lane;
             const bool valid = row_ok && x >= 0 && x < W;
-            const bool f = lane_on(fm[c]);
+            const bool f = lane_on(M[c - col0]);
             if (KIND == SW_FWD) {
                 // infinityNorm(_r) 7:871 (masked in 6+) of the residual the sweep used
                 if (VM ? f : valid) rmax = smax(rmax, fabs(in[e]));
@@ -472,26 +517,6 @@ IFL_DEV void writer_wave(const StreamArgs& a, char* ring, RingHdr* hdr, int j, i
                 const double px = in[512 + e] * pcv, py = in[1024 + e] * pcv;
                 in[512 + e] = f ? px : 0.0;
                 in[1024 + e] = f ? py : 0.0;
-            }
-        }
-        // the boundary row for the neighbouring strip, one step per lane;
-        // the same operations on the same operands as wave 0 performed
-        if (eout && lane < kBlk) {
-            const int t = b * kBlk + lane;
-            const int c = step_col<KIND>(t, cols);
-            const int e = step_in_block<KIND>(lane) * kStrip + (KIND == SW_BWD ? 0 : kStrip - 1);
-            const int x = KIND == SW_BWD ? c : c - (kStrip - 1);
-            if (x >= 0 && x < W) {
-                double v;
-                if (KIND == SW_FWD) {
-                    v = in[2 * 512 + e] * out[e];  // Pn = cy * zm of lane 63
-                } else if (KIND == SW_BWD) {
-                    v = (fm[c] & 1ull) ? out[e] : 0.0;  // zs of lane 0
-                } else {
-                    const double px = in[512 + e], py = in[1024 + e];  // +0 off FLUID
-                    v = ((fm[c] >> 63) & 1ull) ? py * py + tau * px * py : 0.0;
-                }
-                st_relaxed(eout + x, as_bits(v));
             }
         }
         // whole-block stores of the unmasked outputs, 1 KB per instruction
@@ -878,7 +903,11 @@ static int launch_precon(Solver* s, long long* trace) {
     return launch_stream<SW_PRECON, 0, 3, 3>(s, a);
 }
 
-int k_build_precon(Solver* s) { return launch_precon(s, nullptr); }
+int k_build_precon(Solver* s) {
+    IFL_TRY(prof_begin(s, IFL_KERNEL_PRECON, -1));
+    IFL_TRY(launch_precon(s, nullptr));
+    return prof_end(s);
+}
 
 // Diagnostics: one sweep of `kind` on the current state, time-stamped per
 // block (100 MHz clock): wave 0 starts the block, its operands for the
@@ -930,24 +959,31 @@ int k_inf_norm(Solver* s, double* out) {
     return IFL_OK;
 }
 
-// One iteration's kernels; each returns at once when the solve is done.
-static int launch_iteration(Solver* s, int first, int limit) {
+// Iteration `it` of the solve; each kernel returns at once when the solve is
+// already done.
+static int launch_iteration(Solver* s, int it, int limit) {
     VecArgs a = vec_args(s);
     long n = skew_n(s);
     int nbv = grid_blocks(n, 256);
     int seq = s->cfg.dot_order == IFL_DOT_SEQUENTIAL;
-    if (!first) {
+    if (it > 0) {
         vec_update_kernel<<<nbv, 256, 0, s->stream>>>(a, 0, 1);
         IFL_LAUNCH_CHECK();
     }
+    IFL_TRY(prof_begin(s, IFL_KERNEL_MATVEC, it));
     matvec_kernel<1><<<nbv, 256, 0, s->stream>>>(a, 1);
     IFL_LAUNCH_CHECK();
+    IFL_TRY(prof_end(s));
     fin_alpha_kernel<<<1, 256, 0, s->stream>>>(a, nbv, seq);
     IFL_LAUNCH_CHECK();
+    IFL_TRY(prof_begin(s, IFL_KERNEL_SWEEP_FWD, it));
     IFL_TRY(launch_fwd<1>(s));
+    IFL_TRY(prof_end(s));
     fin_check_kernel<<<1, 1, 0, s->stream>>>(s->st, s->cfg.tol);
     IFL_LAUNCH_CHECK();
+    IFL_TRY(prof_begin(s, IFL_KERNEL_SWEEP_BWD, it));
     IFL_TRY(launch_bwd<1>(s));
+    IFL_TRY(prof_end(s));
     fin_sigma_kernel<<<1, 256, 0, s->stream>>>(a, s->sk.nstrips, seq, 0, s->cfg.tol, limit);
     IFL_LAUNCH_CHECK();
     return IFL_OK;
@@ -969,16 +1005,12 @@ int k_project(Solver* s, int limit, ifl_solve_stats* stats) {
     IFL_LAUNCH_CHECK();
     int launched = 0;
     int batch = 4;
-    bool first = true;
     while (true) {
         IFL_HIP(hipMemcpyAsync(s->st_host, s->st, sizeof(PcgState), hipMemcpyDeviceToHost, s->stream));
         IFL_HIP(hipStreamSynchronize(s->stream));
         if (s->st_host->done || s->st_host->timeout || launched >= limit) break;
         int nb = std::min(batch, limit - launched);
-        for (int k = 0; k < nb; k++) {
-            IFL_TRY(launch_iteration(s, first, limit));
-            first = false;
-        }
+        for (int k = 0; k < nb; k++) IFL_TRY(launch_iteration(s, launched + k, limit));
         launched += nb;
         batch = std::min(batch * 2, 64);
     }
@@ -1011,6 +1043,12 @@ int k_project(Solver* s, int limit, ifl_solve_stats* stats) {
         int its = h.status == IFL_SOLVE_CONVERGED ? h.printed_iter + 1 : h.iter;
         pr.iterations += its;
         pr.cell_iterations += (double)its * s->W * s->H;
+        // iterations whose sweeps did work: the converging iteration stops
+        // after its forward sweep
+        int lim[IFL_KERNEL_COUNT] = {0};
+        lim[IFL_KERNEL_MATVEC] = lim[IFL_KERNEL_SWEEP_FWD] = its;
+        lim[IFL_KERNEL_SWEEP_BWD] = h.status == IFL_SOLVE_CONVERGED ? its - 1 : its;
+        IFL_TRY(prof_collect(s, lim));
     } else {
         IFL_HIP(hipStreamSynchronize(s->stream));
     }
