@@ -350,65 +350,91 @@ IFL_DEV void compute_wave(const StreamArgs& a, char* ring, RingHdr* hdr, const d
         }
     };
 
-    double XA[NC][kHalf], XB[NC][kHalf], EA[kHalf], EB[kHalf];
-    uint64_t MA[kHalf], MB[kHalf];
-    // operands of step k of half h
-    auto load_step = [&](const char* slot, int h, int k, double (&X)[NC][kHalf], double (&E)[kHalf], uint64_t (&M)[kHalf]) {
-        const int e = step_in_block<KIND>((h & 1) * kHalf + k) * kStrip + lane;
+    // Operands of a half, shaped as the loads deliver them so that the
+    // loop-carried registers are the load destinations (no copies, which
+    // would wait for the loads): X[ai][p] holds steps 2p and 2p+1 of array
+    // ai (one ds_read2st64), E[p] the two edge values of those steps, M the
+    // four lane masks in sweep-column order (one scalar load).
+    typedef uint64_t vec4u __attribute__((ext_vector_type(4)));
+    typedef const __attribute__((address_space(4))) vec4u const_u64x4;
+    vec2d XA[NC][2], XB[NC][2], EA[2], EB[2];
+    vec4u MA, MB;
+    auto load_half = [&](const char* slot, int h, vec2d (&X)[NC][2], vec2d (&E)[2], vec4u& M) {
+        // past the last half (the unused prefetch) stay on the strip's own
+        // masks and edge ring
+        const int hm = h < nhalf ? h : nhalf - 1;
 #pragma unroll
-        for (int ai = 0; ai < NC; ai++) X[ai][k] = ((const double*)(slot + ai * 4096))[e];
-        E[k] = ering[(h * kHalf + k) % kERing];
-        M[k] = fm[step_col<KIND>(h * kHalf + k, cols)];
+        for (int p = 0; p < 2; p++) {
+            const int e0 = step_in_block<KIND>((h & 1) * kHalf + 2 * p) * kStrip + lane;
+            const int e1 = step_in_block<KIND>((h & 1) * kHalf + 2 * p + 1) * kStrip + lane;
+#pragma unroll
+            for (int ai = 0; ai < NC; ai++) {
+                const double* arr = (const double*)(slot + ai * 4096);
+                X[ai][p].x = arr[e0];
+                X[ai][p].y = arr[e1];
+            }
+            E[p] = *(const vec2d*)(ering + (hm * kHalf + 2 * p) % kERing);
+        }
+        // the 4 columns of the half, lowest first
+        const int c0 = KIND == SW_BWD ? step_col<KIND>(hm * kHalf + kHalf - 1, cols) : step_col<KIND>(hm * kHalf, cols);
+        M = *(const const_u64x4*)(fm + c0);
+    };
+    auto mask_of = [&](const vec4u& M, int k) -> uint64_t {
+        const int i = KIND == SW_BWD ? kHalf - 1 - k : k;
+        return i == 0 ? M.x : i == 1 ? M.y : i == 2 ? M.z : M.w;
     };
     double Pw = 0.0, Pn = 0.0, zs = 0.0;
-    // half h from (X, E, M); while at it, the operands of half h+1 (in nslot) into (Y, F, N)
-    auto run_half = [&](const char* slot, int h, const double (&X)[NC][kHalf], const double (&E)[kHalf],
-                        const uint64_t (&M)[kHalf], const char* nslot, double (&Y)[NC][kHalf], double (&F)[kHalf],
-                        uint64_t (&N)[kHalf]) {
+    // half h from (X, E, M); first the operands of half h+1 (in nslot) into (Y, F, N)
+    auto run_half = [&](const char* slot, int h, const vec2d (&X)[NC][2], const vec2d (&E)[2], const vec4u& M,
+                        const char* nslot, vec2d (&Y)[NC][2], vec2d (&F)[2], vec4u& N) {
         double* out = (double*)(slot + S::NS * 4096);
         // Everything this half uses arrived during the previous half. Drain
         // LDS and scalar loads now, before the next half's reads go out:
         // scalar loads return out of order, so a wait for the masks placed
         // after those reads would wait for all of them.
         __builtin_amdgcn_s_waitcnt(0xC07F);  // lgkmcnt(0)
-        if (nslot) {
-#pragma unroll
-            for (int k = 0; k < kHalf; k++) load_step(nslot, h + 1, k, Y, F, N);
-        }
+        // unconditional (past the last block it reads a valid, unused slot):
+        // a conditional load would merge loaded and unloaded values and cost
+        // register copies that wait for the loads
+        load_half(nslot, h + 1, Y, F, N);
         __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
         for (int k = 0; k < kHalf; k++) {
             const int e = step_in_block<KIND>((h & 1) * kHalf + k) * kStrip + lane;
-            const bool f = lane_on(M[k]);
+            const bool f = lane_on(mask_of(M, k));
+            const int p = k >> 1;
+            const bool hi = k & 1;
+            auto x = [&](int ai) -> double { return hi ? X[ai][p].y : X[ai][p].x; };
+            const double ek = hi ? E[p].y : E[p].x;
             if (KIND == SW_FWD) {
                 // applyPreconditioner forward loop 7:773-787 on the updated residual
-                const double Ps = wave_shr1(Pn, E[k]);
-                const double t1 = X[0][k] - Pw;
+                const double Ps = wave_shr1(Pn, ek);
+                const double t1 = x(0) - Pw;
                 const double tt = t1 - Ps;
-                const double zz = tt * X[3][k];
+                const double zz = tt * x(3);
                 const double zm = f ? zz : 0.0;
-                Pw = X[1][k] * zm;
-                Pn = X[2][k] * zm;
+                Pw = x(1) * zm;
+                Pn = x(2) * zm;
                 out[e] = zm;
             } else if (KIND == SW_BWD) {
                 // applyPreconditioner backward loop 7:789-803
-                const double zn = wave_shl1(zs, E[k]);
-                const double t1 = X[0][k] - X[1][k] * zs;
-                const double tt = t1 - X[2][k] * zn;
-                const double zz = tt * X[3][k];
+                const double zn = wave_shl1(zs, ek);
+                const double t1 = x(0) - x(1) * zs;
+                const double tt = t1 - x(2) * zn;
+                const double zz = tt * x(3);
                 zs = f ? zz : 0.0;
                 out[e] = zz;
             } else {
                 // buildPreconditioner 7:744-767
-                const double Ns = wave_shr1(Pn, E[k]);
-                const double ad = X[0][k];
+                const double Ns = wave_shr1(Pn, ek);
+                const double ad = x(0);
                 double ev = ad;
                 ev = ev - Pw;
                 ev = ev - Ns;
                 if (ev < sigma * ad) ev = ad;
                 const double pcv = 1.0 / sqrt(ev);
-                const double px = X[1][k] * pcv;
-                const double py = X[2][k] * pcv;
+                const double px = x(1) * pcv;
+                const double py = x(2) * pcv;
                 const double eo = px * px + tau * px * py;
                 const double no = py * py + tau * px * py;
                 Pw = f ? eo : 0.0;
@@ -420,8 +446,7 @@ IFL_DEV void compute_wave(const StreamArgs& a, char* ring, RingHdr* hdr, const d
 
     if (!ready(0, kHalf)) return;
     const char* slot = ring;  // block 0 is in slot 0
-#pragma unroll
-    for (int k = 0; k < kHalf; k++) load_step(slot, 0, k, XA, EA, MA);
+    load_half(slot, 0, XA, EA, MA);
     int sidx = 0;
     for (int b = 0; b < nblk; b++) {
         stamp(b, 0);
@@ -436,13 +461,12 @@ IFL_DEV void compute_wave(const StreamArgs& a, char* ring, RingHdr* hdr, const d
         const bool more = b + 1 < nblk;
         if (more && !ready(b + 1, b * kBlk + kBlk + kHalf)) return;
         stamp(b, 3);
-        run_half(slot, 2 * b + 1, XB, EB, MB, more ? nslot : nullptr, XA, EA, MA);
+        run_half(slot, 2 * b + 1, XB, EB, MB, nslot, XA, EA, MA);
         lds_st(&hdr->computed, b + 1);
         stamp(b, 4);
         slot = nslot;
         sidx = nidx;
     }
-    (void)nhalf;
 }
 
 // Wave 2: drains finished blocks.
