@@ -55,6 +55,11 @@ typedef struct ifl_body {
 } ifl_body;
 
 /* ---- configuration ---- */
+/* IFL_PREC_FP64: the drop-in, every operation in double as the reference.
+ * IFL_PREC_MIXED: the same FP64 conjugate-gradient iteration (residual,
+ * iterate, directions, A*s, dot products, alpha/beta and the convergence test
+ * on max|r| in double), with the MIC(0) preconditioner applied in single
+ * precision from a single-precision copy of the double-built factor. */
 enum { IFL_PREC_FP64 = 0, IFL_PREC_MIXED = 1 };
 enum { IFL_DOT_TREE = 0, IFL_DOT_SEQUENTIAL = 1 };
 
@@ -75,9 +80,7 @@ typedef struct ifl_config {
     double t_amb;         /* 294 (7:950) */
     double gravity;       /* 9.81 (7:951) */
     double flip_alpha;    /* 0.001 (8:1296) */
-    double mixed_inner_rtol; /* mixed form: inner FP32 PCG relative reduction per refinement */
-    int32_t mixed_max_outer; /* mixed form: refinement sweeps before giving up */
-    int32_t reserved[7];
+    int32_t reserved[10];
 } ifl_config;
 
 /* Fills cfg with the reference's literal defaults for `level`. */
@@ -96,8 +99,6 @@ typedef struct ifl_solve_stats {
     int32_t iterations;   /* the `iter` the reference prints on convergence,
                              or the limit on budget exhaustion */
     double max_error;     /* final max |r| over the reference's norm domain */
-    int32_t outer;        /* mixed form: refinement sweeps; 0 for FP64 */
-    int32_t inner_total;  /* mixed form: summed inner iterations */
 } ifl_solve_stats;
 
 typedef struct ifl_step_stats {

@@ -161,7 +161,8 @@ void free_all(Solver* s) {
     }
     F(s->udens); F(s->vdens);
     F(s->r); F(s->p); F(s->z); F(s->s); F(s->precon); F(s->adiag); F(s->aplusx); F(s->aplusy);
-    F(s->cx); F(s->cy); F(s->yv); F(s->qv); F(s->dflag); F(s->fmask); F(s->edge); F(s->partial); F(s->st);
+    F(s->cx); F(s->cy); F(s->yv); F(s->qv); F(s->dflag); F(s->fmask); F(s->edge);
+    F(s->cx32); F(s->cy32); F(s->pc32); F(s->y32); F(s->partial); F(s->st);
     F(s->stage_buf); F(s->front); F(s->front_cnt); F(s->bodies_d);
     F(s->posx); F(s->posy);
     for (int t = 0; t < 4; t++) F(s->prop[t]);
@@ -232,6 +233,13 @@ int init_solver(Solver* s) {
                        &s->cx, &s->cy, &s->yv, &s->qv};
     for (double** a : arrs) IFL_TRY(dalloc(s, a, n));
     IFL_TRY(balloc(s, &s->dflag, n));
+    if (s->cfg.precision == IFL_PREC_MIXED) {
+        float** sarrs[] = {&s->cx32, &s->cy32, &s->pc32, &s->y32};
+        for (float** a : sarrs) {
+            IFL_HIP(hipMalloc((void**)a, n * sizeof(float)));
+            IFL_HIP(hipMemsetAsync(*a, 0, n * sizeof(float), s->stream));
+        }
+    }
     IFL_HIP(hipMalloc((void**)&s->fmask, (size_t)sk.nstrips * sk.cols * sizeof(uint64_t)));
     IFL_HIP(hipMemsetAsync(s->fmask, 0, (size_t)sk.nstrips * sk.cols * sizeof(uint64_t), s->stream));
     long ne = 3L * sk.nstrips * W;
@@ -291,7 +299,7 @@ int fill_solid(Solver* s, int qi) {
 // 6:966-1008, 7:995-1034)
 int update_grid(Solver* s, double dt, ifl_step_stats* stats) {
     int lv = s->level;
-    ifl_solve_stats heat = {IFL_SOLVE_NOT_RUN, 0, 0.0, 0, 0}, press = heat;
+    ifl_solve_stats heat = {IFL_SOLVE_NOT_RUN, 0, 0.0}, press = heat;
     s->border_solid = 0;
     if (lv == 3) {
         IFL_TRY(k_build_rhs(s));
@@ -378,8 +386,6 @@ int ifl_config_default(ifl_config* cfg, int level, int w, int h) {
     cfg->t_amb = 294.0;          // 7:950
     cfg->gravity = 9.81;         // 7:951
     cfg->flip_alpha = 0.001;     // 8:1296
-    cfg->mixed_inner_rtol = 1e-3;
-    cfg->mixed_max_outer = 64;
     return IFL_OK;
 }
 
@@ -394,9 +400,9 @@ int ifl_create(const ifl_config* cfg, ifl_solver** out) {
         g_err = "ifl_create: level 8 (FLIP) is not available in this build";
         return IFL_ERR_UNSUPPORTED;
     }
-    if (cfg->precision != IFL_PREC_FP64) {
-        g_err = "ifl_create: only the FP64 form is available in this build";
-        return IFL_ERR_UNSUPPORTED;
+    if (cfg->precision != IFL_PREC_FP64 && cfg->precision != IFL_PREC_MIXED) {
+        g_err = "ifl_create: precision must be IFL_PREC_FP64 or IFL_PREC_MIXED";
+        return IFL_ERR_ARG;
     }
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= cfg->device) {

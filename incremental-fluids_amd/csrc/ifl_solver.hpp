@@ -58,6 +58,8 @@ struct Solver {
     double *precon = nullptr, *adiag = nullptr, *aplusx = nullptr, *aplusy = nullptr;
     double *cx = nullptr, *cy = nullptr;  // aPlusX*precon, aPlusY*precon (the sweeps' products)
     double *yv = nullptr;                 // forward-sweep output
+    // mixed form: the factor and the forward-sweep output in single precision
+    float *cx32 = nullptr, *cy32 = nullptr, *pc32 = nullptr, *y32 = nullptr;
     double *qv = nullptr;                 // A*s
     uint8_t* dflag = nullptr;             // 1 where the d-cell is FLUID (and inside the grid)
     uint64_t* fmask = nullptr;            // dflag as one 64-lane mask per strip column (nstrips*cols)

@@ -79,7 +79,7 @@ constexpr int kLoaders = 5;
 constexpr int kWaves = 3 + kLoaders;
 constexpr int kERing = 512;  // edge ring, in steps
 constexpr int kMaxIn = 6;
-constexpr int kMaxOut = 3;
+constexpr int kMaxOut = 6;
 constexpr long kLdsSpin = 1L << 22;
 
 enum { SW_FWD = 0, SW_BWD = 1, SW_PRECON = 2 };
@@ -98,17 +98,17 @@ static_assert(sizeof(RingHdr) == 64, "RingHdr layout");
 
 struct StreamArgs {
     Skew sk;
-    const double* in[kMaxIn];
+    const void* in[kMaxIn];
     const uint64_t* fmask;
     double* edge;           // this sweep's edge buffers (nstrips * W)
-    double* out[kMaxOut];   // fwd: y, r   bwd: z   precon: precon, cx, cy
+    void* out[kMaxOut];     // fwd: y, r   bwd: z   precon: precon, cx, cy (+ their single-precision copies)
     double* partial;
     PcgState* st;
     int* timeout;
     double tau, sigma;
     int strip0, nlaunch;    // the strips of this launch
     int nb;                 // ring depth in blocks
-    int slot_bytes;         // (inputs + outputs) * 4096
+    int slot_bytes;         // bytes of one ring slot
     long long* trace;       // diagnostics: kTraceStamps stamps per strip and block, or null
 };
 
@@ -185,16 +185,41 @@ IFL_DEV int edge_x(int t, int cols) {
 }
 
 typedef double vec2d __attribute__((ext_vector_type(2)));
+typedef float vec2f __attribute__((ext_vector_type(2)));
+typedef int vec4i __attribute__((ext_vector_type(4)));
+template <typename T> struct Vec2;
+template <> struct Vec2<double> { typedef vec2d type; };
+template <> struct Vec2<float> { typedef vec2f type; };
 
-// Per sweep: NG arrays streamed from HBM, NS of them kept in the LDS slot
-// (the forward update folds q into r on the way), NC read by wave 0, and
-// one output array written by wave 0.
-template <int KIND, int MODE, int NG>
+IFL_DEV double lane_shr1(double v, double edge) { return wave_shr1(v, edge); }
+IFL_DEV float lane_shr1(float v, float edge) { return wave_shr1f(v, edge); }
+IFL_DEV double lane_shl1(double v, double edge) { return wave_shl1(v, edge); }
+IFL_DEV float lane_shl1(float v, float edge) { return wave_shl1f(v, edge); }
+
+// Layout of one sweep. T is the precision of the triangular solves: double
+// in the drop-in, float in the mixed form (the factor build is always
+// double). NG arrays are streamed from HBM; NS of them are kept in the LDS
+// slot (the forward update folds q into r on the way). The residual and the
+// vectors the writer reduces with stay double; the factor (cx, cy, precon)
+// and the forward result y are T. Each array occupies 512*es bytes of a
+// slot, followed by wave 0's output (512 elements).
+template <int KIND, int MODE, int NG, typename T>
 struct SweepShape {
     static constexpr int NS = NG - (KIND == SW_FWD && MODE == 1 ? 1 : 0);
-    static constexpr int NC = KIND == SW_PRECON ? 3 : 4;
-    static constexpr int R = NG <= 4 ? 3 : 2;  // blocks in flight per loader, NG*16 VGPRs each
-    static constexpr int SLOT = (NS + 1) * 4096;
+    static constexpr int es(int ai) {
+        return KIND == SW_FWD ? ((ai == 0 || ai == 4) ? 8 : (int)sizeof(T))
+                              : (KIND == SW_BWD ? (ai <= 3 ? (int)sizeof(T) : 8) : 8);
+    }
+    static constexpr int rb(int ai) { return 512 * es(ai); }
+    static constexpr int ro(int ai) { return ai == 0 ? 0 : ro(ai - 1) + rb(ai - 1); }
+    static constexpr int lo(int ai) { return ai == 0 ? 0 : lo(ai - 1) + rb(ai - 1) / 1024; }
+    static constexpr int NLD = lo(NG);  // 1 KB loads per block
+    static constexpr int OUT = ro(NS);  // output region
+    static constexpr int SLOT = OUT + 512 * (KIND == SW_PRECON ? 8 : (int)sizeof(T));
+    static constexpr int R = NLD <= 16 ? 3 : 2;  // blocks in flight per loader, 4 VGPRs per load
+    // arrays wave 0 reads in precision T, and where they are in the slot
+    static constexpr int NT = KIND == SW_BWD ? 4 : 3;
+    static constexpr int tarr(int i) { return KIND == SW_FWD ? i + 1 : i; }
 };
 
 IFL_DEV bool lane_on(uint64_t m) { return __builtin_amdgcn_inverse_ballot_w64(m); }
@@ -207,39 +232,41 @@ IFL_DEV const const_u64* strip_masks(const StreamArgs& a, int j) {
 }
 
 // Each loader streams blocks k, k + kLoaders, ... . Up to R blocks are in
-// flight in its registers (global_load_dwordx4: lane i holds elements
-// 2i, 2i+1 of each 128-element run), so the prefetch depth does not depend
-// on free LDS slots; a block is written into its slot once the slot is free.
-// In the forward sweep with the residual update (MODE 1) the loader forms
-// r + q*(-alpha) (scaledAdd 7:870; at FLUID cells only from level 6 on) and
-// keeps only the result.
-template <int KIND, int MODE, int LV, int NG>
+// flight in its registers (global_load_dwordx4: lane i holds bytes
+// [16 i, 16 i + 16) of each 1 KB run of an array's block), so the prefetch
+// depth does not depend on free LDS slots; a block is written into its slot
+// once the slot is free. In the forward sweep with the residual update
+// (MODE 1) the loader forms r + q*(-alpha) (scaledAdd 7:870; at FLUID cells
+// only from level 6 on) and keeps only the result.
+template <int KIND, int MODE, int LV, int NG, typename T>
 IFL_DEV void loader_wave(const StreamArgs& a, char* ring, RingHdr* hdr, int k, int lane, long sbase, int nblk, int j) {
-    typedef SweepShape<KIND, MODE, NG> S;
-    constexpr int R = S::R, NL = NG * 4;
+    typedef SweepShape<KIND, MODE, NG, T> S;
+    constexpr int R = S::R, NL = S::NLD;
     constexpr bool UPD = KIND == SW_FWD && MODE == 1;
     const int cols = a.sk.cols;
     const int nmine = nblk > k ? (nblk - k + kLoaders - 1) / kLoaders : 0;
     const double alpha_neg = UPD ? -a.st->alpha : 0.0;
     const const_u64* fm = strip_masks(a, j);
-    vec2d b0[NL], b1[NL], b2[NL];
-    auto issue = [&](int i, vec2d* B) __attribute__((always_inline)) {
-        const long off = sbase + (long)block_col0<KIND>(k + i * kLoaders, cols) * kStrip + lane * 2;
+    vec4i b0[NL], b1[NL], b2[NL];
+    auto issue = [&](int i, vec4i* B) __attribute__((always_inline)) {
+        const long e0 = sbase + (long)block_col0<KIND>(k + i * kLoaders, cols) * kStrip;
 #pragma unroll
-        for (int ai = 0; ai < NG; ai++)
+        for (int ai = 0; ai < NG; ai++) {
+            const char* p = (const char*)a.in[ai] + e0 * S::es(ai) + lane * 16;
 #pragma unroll
-            for (int m = 0; m < 4; m++) B[ai * 4 + m] = *(const vec2d*)(a.in[ai] + off + m * 128);
+            for (int m = 0; m < S::rb(ai) / 1024; m++) B[S::lo(ai) + m] = *(const vec4i*)(p + m * 1024);
+        }
     };
     // land block i (from B) in its slot, then reuse B for block i + R
-    auto stage = [&](int i, vec2d* B) __attribute__((always_inline)) -> bool {
+    auto stage = [&](int i, vec4i* B) __attribute__((always_inline)) -> bool {
         if (i >= nmine) return true;
         const int b = k + i * kLoaders;
         if (UPD) {
             const int col0 = block_col0<KIND>(b, cols);
 #pragma unroll
             for (int m = 0; m < 4; m++) {
-                const vec2d r = B[m];
-                vec2d rn = r + B[16 + m] * alpha_neg;
+                const vec2d r = __builtin_bit_cast(vec2d, B[m]);
+                vec2d rn = r + __builtin_bit_cast(vec2d, B[S::lo(4) + m]) * alpha_neg;
                 if (LV >= 6) {
                     // element 2*lane + c of run m is step 2m + lane/32, row 2*(lane%32) + c
                     const uint64_t w = lane < 32 ? fm[col0 + 2 * m] : fm[col0 + 2 * m + 1];
@@ -247,7 +274,7 @@ IFL_DEV void loader_wave(const StreamArgs& a, char* ring, RingHdr* hdr, int k, i
                     rn.x = (bits & 1u) ? rn.x : r.x;
                     rn.y = (bits & 2u) ? rn.y : r.y;
                 }
-                B[m] = rn;
+                B[m] = __builtin_bit_cast(vec4i, rn);
             }
         }
         if (!lds_wait_gt(&hdr->consumed, b - a.nb, hdr, a.timeout)) return false;
@@ -255,7 +282,8 @@ IFL_DEV void loader_wave(const StreamArgs& a, char* ring, RingHdr* hdr, int k, i
 #pragma unroll
         for (int ai = 0; ai < S::NS; ai++)
 #pragma unroll
-            for (int m = 0; m < 4; m++) *(vec2d*)(slot + ai * 4096 + m * 1024 + lane * 16) = B[ai * 4 + m];
+            for (int m = 0; m < S::rb(ai) / 1024; m++)
+                *(vec4i*)(slot + S::ro(ai) + m * 1024 + lane * 16) = B[S::lo(ai) + m];
         lds_st(&hdr->ready[k], b + 1);
         if (i + R < nmine) issue(i + R, B);
         return true;
@@ -309,10 +337,11 @@ IFL_DEV void poller_wave(const StreamArgs& a, const double* ein, double* ering, 
 
 // Wave 0. Operand registers are double-buffered by half block (4 steps):
 // a half first issues the LDS reads of the next half, then runs its steps.
-template <int KIND, int MODE, int NG>
+template <int KIND, int MODE, int NG, typename T>
 IFL_DEV void compute_wave(const StreamArgs& a, char* ring, RingHdr* hdr, const double* ering, int j, int lane) {
-    typedef SweepShape<KIND, MODE, NG> S;
-    constexpr int NC = S::NC;
+    typedef SweepShape<KIND, MODE, NG, T> S;
+    typedef typename Vec2<T>::type V2;
+    constexpr int NT = S::NT;
     const int cols = a.sk.cols, nblk = cols / kBlk, nhalf = 2 * nblk;
     const const_u64* fm = strip_masks(a, j);
     const double tau = a.tau, sigma = a.sigma;
@@ -352,14 +381,16 @@ IFL_DEV void compute_wave(const StreamArgs& a, char* ring, RingHdr* hdr, const d
 
     // Operands of a half, shaped as the loads deliver them so that the
     // loop-carried registers are the load destinations (no copies, which
-    // would wait for the loads): X[ai][p] holds steps 2p and 2p+1 of array
-    // ai (one ds_read2st64), E[p] the two edge values of those steps, M the
-    // four lane masks in sweep-column order (one scalar load).
+    // would wait for the loads): X[i][p] holds steps 2p and 2p+1 of the i-th
+    // T array (one ds_read2st64), Q[p] those of the updated residual (forward
+    // sweep), E[p] the two edge values of those steps, M the four lane masks
+    // in sweep-column order (one scalar load).
     typedef uint64_t vec4u __attribute__((ext_vector_type(4)));
     typedef const __attribute__((address_space(4))) vec4u const_u64x4;
-    vec2d XA[NC][2], XB[NC][2], EA[2], EB[2];
+    V2 XA[NT][2], XB[NT][2];
+    vec2d QA[2], QB[2], EA[2], EB[2];
     vec4u MA, MB;
-    auto load_half = [&](const char* slot, int h, vec2d (&X)[NC][2], vec2d (&E)[2], vec4u& M) {
+    auto load_half = [&](const char* slot, int h, V2 (&X)[NT][2], vec2d (&Q)[2], vec2d (&E)[2], vec4u& M) {
         // past the last half (the unused prefetch) stay on the strip's own
         // masks and edge ring
         const int hm = h < nhalf ? h : nhalf - 1;
@@ -368,10 +399,15 @@ IFL_DEV void compute_wave(const StreamArgs& a, char* ring, RingHdr* hdr, const d
             const int e0 = step_in_block<KIND>((h & 1) * kHalf + 2 * p) * kStrip + lane;
             const int e1 = step_in_block<KIND>((h & 1) * kHalf + 2 * p + 1) * kStrip + lane;
 #pragma unroll
-            for (int ai = 0; ai < NC; ai++) {
-                const double* arr = (const double*)(slot + ai * 4096);
-                X[ai][p].x = arr[e0];
-                X[ai][p].y = arr[e1];
+            for (int i = 0; i < NT; i++) {
+                const T* arr = (const T*)(slot + S::ro(S::tarr(i)));
+                X[i][p].x = arr[e0];
+                X[i][p].y = arr[e1];
+            }
+            if (KIND == SW_FWD) {
+                const double* arr = (const double*)slot;
+                Q[p].x = arr[e0];
+                Q[p].y = arr[e1];
             }
             E[p] = *(const vec2d*)(ering + (hm * kHalf + 2 * p) % kERing);
         }
@@ -383,11 +419,11 @@ IFL_DEV void compute_wave(const StreamArgs& a, char* ring, RingHdr* hdr, const d
         const int i = KIND == SW_BWD ? kHalf - 1 - k : k;
         return i == 0 ? M.x : i == 1 ? M.y : i == 2 ? M.z : M.w;
     };
-    double Pw = 0.0, Pn = 0.0, zs = 0.0;
-    // half h from (X, E, M); first the operands of half h+1 (in nslot) into (Y, F, N)
-    auto run_half = [&](const char* slot, int h, const vec2d (&X)[NC][2], const vec2d (&E)[2], const vec4u& M,
-                        const char* nslot, vec2d (&Y)[NC][2], vec2d (&F)[2], vec4u& N) {
-        double* out = (double*)(slot + S::NS * 4096);
+    T Pw = 0, Pn = 0, zs = 0;
+    // half h from (X, Q, E, M); first the operands of half h+1 (in nslot) into (Y, P, F, N)
+    auto run_half = [&](const char* slot, int h, const V2 (&X)[NT][2], const vec2d (&Q)[2], const vec2d (&E)[2],
+                        const vec4u& M, const char* nslot, V2 (&Y)[NT][2], vec2d (&P)[2], vec2d (&F)[2], vec4u& N) {
+        T* out = (T*)(slot + S::OUT);
         // Everything this half uses arrived during the previous half. Drain
         // LDS and scalar loads now, before the next half's reads go out:
         // scalar loads return out of order, so a wait for the masks placed
@@ -396,7 +432,7 @@ IFL_DEV void compute_wave(const StreamArgs& a, char* ring, RingHdr* hdr, const d
         // unconditional (past the last block it reads a valid, unused slot):
         // a conditional load would merge loaded and unloaded values and cost
         // register copies that wait for the loads
-        load_half(nslot, h + 1, Y, F, N);
+        load_half(nslot, h + 1, Y, P, F, N);
         __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
         for (int k = 0; k < kHalf; k++) {
@@ -404,29 +440,29 @@ IFL_DEV void compute_wave(const StreamArgs& a, char* ring, RingHdr* hdr, const d
             const bool f = lane_on(mask_of(M, k));
             const int p = k >> 1;
             const bool hi = k & 1;
-            auto x = [&](int ai) -> double { return hi ? X[ai][p].y : X[ai][p].x; };
-            const double ek = hi ? E[p].y : E[p].x;
+            auto x = [&](int i) -> T { return hi ? X[i][p].y : X[i][p].x; };
+            const T ek = (T)(hi ? E[p].y : E[p].x);
             if (KIND == SW_FWD) {
                 // applyPreconditioner forward loop 7:773-787 on the updated residual
-                const double Ps = wave_shr1(Pn, ek);
-                const double t1 = x(0) - Pw;
-                const double tt = t1 - Ps;
-                const double zz = tt * x(3);
-                const double zm = f ? zz : 0.0;
-                Pw = x(1) * zm;
-                Pn = x(2) * zm;
+                const T Ps = lane_shr1(Pn, ek);
+                const T t1 = (T)(hi ? Q[p].y : Q[p].x) - Pw;
+                const T tt = t1 - Ps;
+                const T zz = tt * x(2);
+                const T zm = f ? zz : (T)0;
+                Pw = x(0) * zm;
+                Pn = x(1) * zm;
                 out[e] = zm;
             } else if (KIND == SW_BWD) {
                 // applyPreconditioner backward loop 7:789-803
-                const double zn = wave_shl1(zs, ek);
-                const double t1 = x(0) - x(1) * zs;
-                const double tt = t1 - x(2) * zn;
-                const double zz = tt * x(3);
-                zs = f ? zz : 0.0;
+                const T zn = lane_shl1(zs, ek);
+                const T t1 = x(0) - x(1) * zs;
+                const T tt = t1 - x(2) * zn;
+                const T zz = tt * x(3);
+                zs = f ? zz : (T)0;
                 out[e] = zz;
             } else {
-                // buildPreconditioner 7:744-767
-                const double Ns = wave_shr1(Pn, ek);
+                // buildPreconditioner 7:744-767 (T is double)
+                const double Ns = lane_shr1((double)Pn, (double)ek);
                 const double ad = x(0);
                 double ev = ad;
                 ev = ev - Pw;
@@ -446,14 +482,14 @@ IFL_DEV void compute_wave(const StreamArgs& a, char* ring, RingHdr* hdr, const d
 
     if (!ready(0, kHalf)) return;
     const char* slot = ring;  // block 0 is in slot 0
-    load_half(slot, 0, XA, EA, MA);
+    load_half(slot, 0, XA, QA, EA, MA);
     int sidx = 0;
     for (int b = 0; b < nblk; b++) {
         stamp(b, 0);
         // second half of this block: same slot
         if (!ready(b, b * kBlk + kBlk)) return;
         stamp(b, 1);
-        run_half(slot, 2 * b, XA, EA, MA, slot, XB, EB, MB);
+        run_half(slot, 2 * b, XA, QA, EA, MA, slot, XB, QB, EB, MB);
         stamp(b, 2);
         // first half of the next block
         const int nidx = sidx + 1 == a.nb ? 0 : sidx + 1;
@@ -461,7 +497,7 @@ IFL_DEV void compute_wave(const StreamArgs& a, char* ring, RingHdr* hdr, const d
         const bool more = b + 1 < nblk;
         if (more && !ready(b + 1, b * kBlk + kBlk + kHalf)) return;
         stamp(b, 3);
-        run_half(slot, 2 * b + 1, XB, EB, MB, nslot, XA, EA, MA);
+        run_half(slot, 2 * b + 1, XB, QB, EB, MB, nslot, XA, QA, EA, MA);
         lds_st(&hdr->computed, b + 1);
         stamp(b, 4);
         slot = nslot;
@@ -470,9 +506,9 @@ IFL_DEV void compute_wave(const StreamArgs& a, char* ring, RingHdr* hdr, const d
 }
 
 // Wave 2: drains finished blocks.
-template <int KIND, int MODE, int LV, int NG>
+template <int KIND, int MODE, int LV, int NG, typename T>
 IFL_DEV void writer_wave(const StreamArgs& a, char* ring, RingHdr* hdr, int j, int lane, double* eout) {
-    typedef SweepShape<KIND, MODE, NG> S;
+    typedef SweepShape<KIND, MODE, NG, T> S;
     constexpr bool VM = LV >= 6;   // vector operations masked to FLUID cells
     constexpr bool AUX = LV == 4;  // levels 4-5: the dot reads the reference's _z at non-FLUID cells
     const int W = a.sk.W, H = a.sk.H, cols = a.sk.cols, nblk = cols / kBlk;
@@ -484,8 +520,8 @@ IFL_DEV void writer_wave(const StreamArgs& a, char* ring, RingHdr* hdr, int j, i
     for (int b = 0; b < nblk; b++) {
         if (!lds_wait_gt(&hdr->computed, b, hdr, a.timeout)) return;
         char* slot = ring + (long)(b % a.nb) * a.slot_bytes;
-        double* in = (double*)slot;
-        const double* out = (const double*)(slot + S::NS * 4096);
+        auto in = [&](int ai) { return slot + S::ro(ai); };  // region of streamed array ai
+        const T* out = (const T*)(slot + S::OUT);
         const int col0 = block_col0<KIND>(b, cols);
         const long gblk = sbase + (long)col0 * kStrip;  // first element of the block in HBM
         // the block's 8 masks in one scalar load, before any of them is used
@@ -507,11 +543,13 @@ IFL_DEV void writer_wave(const StreamArgs& a, char* ring, RingHdr* hdr, int j, i
             if (lane < kBlk && x >= 0 && x < W) {
                 double v;
                 if (KIND == SW_FWD) {
-                    v = in[2 * 512 + e] * out[e];  // Pn = cy * zm of lane 63
+                    const T pn = ((const T*)in(2))[e] * out[e];  // Pn = cy * zm of lane 63
+                    v = (double)pn;
                 } else if (KIND == SW_BWD) {
-                    v = (mc & 1ull) ? out[e] : 0.0;  // zs of lane 0
+                    v = (mc & 1ull) ? (double)out[e] : 0.0;  // zs of lane 0
                 } else {
-                    const double px = in[512 + e] * out[e], py = in[1024 + e] * out[e];
+                    const double px = ((const double*)in(1))[e] * (double)out[e];
+                    const double py = ((const double*)in(2))[e] * (double)out[e];
                     v = ((mc >> 63) & 1ull) ? py * py + tau * px * py : 0.0;
                 }
                 st_relaxed(eout + x, as_bits(v));
@@ -527,36 +565,42 @@ IFL_DEV void writer_wave(const StreamArgs& a, char* ring, RingHdr* hdr, int j, i
             const bool f = lane_on(M[c - col0]);
             if (KIND == SW_FWD) {
                 // infinityNorm(_r) 7:871 (masked in 6+) of the residual the sweep used
-                if (VM ? f : valid) rmax = smax(rmax, fabs(in[e]));
+                if (VM ? f : valid) rmax = smax(rmax, fabs(((const double*)in(0))[e]));
             } else if (KIND == SW_BWD) {
                 // store _z at FLUID cells; dotProduct(_z, _r) 7:880 (unmasked in 3-5)
-                const double zz = out[e];
-                if (f) a.out[0][gblk + e] = zz;
-                const double zdot = f ? zz : (AUX ? in[5 * 512 + e] : 0.0);
-                if (VM ? f : valid) acc += zdot * in[4 * 512 + e];
+                const double zz = (double)out[e];
+                if (f) ((double*)a.out[0])[gblk + e] = zz;
+                const double zdot = f ? zz : (AUX ? ((const double*)in(5))[e] : 0.0);
+                if (VM ? f : valid) acc += zdot * ((const double*)in(4))[e];
             } else {
                 // the factor at FLUID cells; cx = aPlusX*precon, cy likewise, +0 elsewhere
-                const double pcv = out[e];
-                if (f) a.out[0][gblk + e] = pcv;
-                const double px = in[512 + e] * pcv, py = in[1024 + e] * pcv;
-                in[512 + e] = f ? px : 0.0;
-                in[1024 + e] = f ? py : 0.0;
+                const double pcv = (double)out[e];
+                double* apx = (double*)in(1);
+                double* apy = (double*)in(2);
+                if (f) ((double*)a.out[0])[gblk + e] = pcv;
+                const double px = apx[e] * pcv, py = apy[e] * pcv;
+                apx[e] = f ? px : 0.0;
+                apy[e] = f ? py : 0.0;
+                if (a.out[3]) {  // the mixed form's single-precision copies
+                    ((float*)a.out[3])[gblk + e] = f ? (float)pcv : 0.0f;
+                    ((float*)a.out[4])[gblk + e] = (float)apx[e];
+                    ((float*)a.out[5])[gblk + e] = (float)apy[e];
+                }
             }
         }
         // whole-block stores of the unmasked outputs, 1 KB per instruction
-        auto store_block = [&](double* dst, const double* src) {
+        auto store_block = [&](void* dst, const char* src, int es) {
+            char* d = (char*)dst + gblk * es + lane * 16;
 #pragma unroll
-            for (int m = 0; m < 4; m++) {
-                const vec2d v = ((const vec2d*)src)[m * 64 + lane];
-                *(vec2d*)(dst + gblk + m * 128 + lane * 2) = v;
-            }
+            for (int m = 0; m < 4; m++)
+                if (m * 1024 < 512 * es) *(vec4i*)(d + m * 1024) = *(const vec4i*)(src + m * 1024 + lane * 16);
         };
         if (KIND == SW_FWD) {
-            store_block(a.out[0], out);
-            if (MODE == 1) store_block(a.out[1], in);
+            store_block(a.out[0], (const char*)out, sizeof(T));
+            if (MODE == 1) store_block(a.out[1], in(0), 8);
         } else if (KIND == SW_PRECON) {
-            store_block(a.out[1], in + 512);
-            store_block(a.out[2], in + 1024);
+            store_block(a.out[1], in(1), 8);
+            store_block(a.out[2], in(2), 8);
         }
         if (a.trace && lane == 0) a.trace[((long)j * nblk + b) * kTraceStamps + 5] = __builtin_amdgcn_s_memrealtime();
         lds_st(&hdr->consumed, b + 1);
@@ -570,7 +614,7 @@ IFL_DEV void writer_wave(const StreamArgs& a, char* ring, RingHdr* hdr, int j, i
     }
 }
 
-template <int KIND, int MODE, int LV, int NG>
+template <int KIND, int MODE, int LV, int NG, typename T>
 __global__ __launch_bounds__(64 * kWaves) void stream_sweep_kernel(StreamArgs a) {
     extern __shared__ __align__(16) char smem[];
     const int j = strip_of_block(blockIdx.x, a.strip0, a.nlaunch);
@@ -593,13 +637,13 @@ __global__ __launch_bounds__(64 * kWaves) void stream_sweep_kernel(StreamArgs a)
     double* eout = has_out ? a.edge + (long)(KIND == SW_BWD ? j : j + 1) * W : nullptr;
 
     if (wave >= 3)
-        loader_wave<KIND, MODE, LV, NG>(a, ring, hdr, wave - 3, lane, sbase, nblk, j);
+        loader_wave<KIND, MODE, LV, NG, T>(a, ring, hdr, wave - 3, lane, sbase, nblk, j);
     else if (wave == 2)
-        writer_wave<KIND, MODE, LV, NG>(a, ring, hdr, j, lane, eout);
+        writer_wave<KIND, MODE, LV, NG, T>(a, ring, hdr, j, lane, eout);
     else if (wave == 1)
         poller_wave<KIND>(a, ein, ering, hdr, lane);
     else
-        compute_wave<KIND, MODE, NG>(a, ring, hdr, ering, j, lane);
+        compute_wave<KIND, MODE, NG, T>(a, ring, hdr, ering, j, lane);
 }
 
 // ---------------------------------------------------------------- vector phases
@@ -859,9 +903,9 @@ static StreamArgs stream_args(Solver* s, int kind) {
     return a;
 }
 
-template <int KIND, int MODE, int LV, int NA>
+template <int KIND, int MODE, int LV, int NG, typename T>
 static int launch_stream(Solver* s, StreamArgs a) {
-    a.slot_bytes = SweepShape<KIND, MODE, NA>::SLOT;
+    a.slot_bytes = SweepShape<KIND, MODE, NG, T>::SLOT;
     size_t fixed = sizeof(RingHdr) + kERing * sizeof(double);
     int nb = (int)((s->lds_bytes - fixed) / a.slot_bytes);
     if (nb > 12) nb = 12;
@@ -873,7 +917,7 @@ static int launch_stream(Solver* s, StreamArgs a) {
     size_t shmem = fixed + (size_t)nb * a.slot_bytes;
     static bool attr_set = false;
     if (!attr_set) {
-        IFL_HIP(hipFuncSetAttribute((const void*)stream_sweep_kernel<KIND, MODE, LV, NA>,
+        IFL_HIP(hipFuncSetAttribute((const void*)stream_sweep_kernel<KIND, MODE, LV, NG, T>,
                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)shmem));
         attr_set = true;
     }
@@ -884,35 +928,57 @@ static int launch_stream(Solver* s, StreamArgs a) {
     for (int done = 0; done < ns; done += chunk) {
         a.nlaunch = std::min(chunk, ns - done);
         a.strip0 = KIND == SW_BWD ? ns - done - a.nlaunch : done;
-        stream_sweep_kernel<KIND, MODE, LV, NA><<<sweep_blocks(a.nlaunch), 64 * kWaves, shmem, s->stream>>>(a);
+        stream_sweep_kernel<KIND, MODE, LV, NG, T><<<sweep_blocks(a.nlaunch), 64 * kWaves, shmem, s->stream>>>(a);
         IFL_LAUNCH_CHECK();
     }
     return IFL_OK;
 }
 
-template <int MODE>
-static int launch_fwd(Solver* s, long long* trace = nullptr) {
+static bool mixed(const Solver* s) { return s->cfg.precision == IFL_PREC_MIXED; }
+
+// forward sweep: the drop-in reads the double factor, the mixed form its
+// single-precision copy (and writes y in single precision)
+template <int MODE, typename T>
+static int launch_fwd_t(Solver* s, long long* trace) {
     StreamArgs a = stream_args(s, SW_FWD);
-    a.in[0] = s->r; a.in[1] = s->cx; a.in[2] = s->cy; a.in[3] = s->precon; a.in[4] = s->qv;
-    a.out[0] = s->yv;
+    const bool sp = sizeof(T) == 4;
+    a.in[0] = s->r;
+    a.in[1] = sp ? (const void*)s->cx32 : s->cx;
+    a.in[2] = sp ? (const void*)s->cy32 : s->cy;
+    a.in[3] = sp ? (const void*)s->pc32 : s->precon;
+    a.in[4] = s->qv;
+    a.out[0] = sp ? (void*)s->y32 : s->yv;
     a.out[1] = s->r;
     a.trace = trace;
-    constexpr int NA = MODE == 1 ? 5 : 4;
-    if (s->level == 3) return launch_stream<SW_FWD, MODE, 3, NA>(s, a);
-    if (s->level <= 5) return launch_stream<SW_FWD, MODE, 4, NA>(s, a);
-    return launch_stream<SW_FWD, MODE, 6, NA>(s, a);
+    constexpr int NG = MODE == 1 ? 5 : 4;
+    if (s->level == 3) return launch_stream<SW_FWD, MODE, 3, NG, T>(s, a);
+    if (s->level <= 5) return launch_stream<SW_FWD, MODE, 4, NG, T>(s, a);
+    return launch_stream<SW_FWD, MODE, 6, NG, T>(s, a);
+}
+template <int MODE>
+static int launch_fwd(Solver* s, long long* trace = nullptr) {
+    return mixed(s) ? launch_fwd_t<MODE, float>(s, trace) : launch_fwd_t<MODE, double>(s, trace);
 }
 
-template <int MODE>
-static int launch_bwd(Solver* s, long long* trace = nullptr) {
+template <int MODE, typename T>
+static int launch_bwd_t(Solver* s, long long* trace) {
     StreamArgs a = stream_args(s, SW_BWD);
-    a.in[0] = s->yv; a.in[1] = s->cx; a.in[2] = s->cy; a.in[3] = s->precon; a.in[4] = s->r;
+    const bool sp = sizeof(T) == 4;
+    a.in[0] = sp ? (const void*)s->y32 : s->yv;
+    a.in[1] = sp ? (const void*)s->cx32 : s->cx;
+    a.in[2] = sp ? (const void*)s->cy32 : s->cy;
+    a.in[3] = sp ? (const void*)s->pc32 : s->precon;
+    a.in[4] = s->r;
     a.in[5] = MODE == 1 ? s->qv : s->z;  // the reference's _z at non-FLUID cells (levels 4-5 dot)
     a.out[0] = s->z;
     a.trace = trace;
-    if (s->level == 3) return launch_stream<SW_BWD, MODE, 3, 5>(s, a);
-    if (s->level <= 5) return launch_stream<SW_BWD, MODE, 4, 6>(s, a);
-    return launch_stream<SW_BWD, MODE, 6, 5>(s, a);
+    if (s->level == 3) return launch_stream<SW_BWD, MODE, 3, 5, T>(s, a);
+    if (s->level <= 5) return launch_stream<SW_BWD, MODE, 4, 6, T>(s, a);
+    return launch_stream<SW_BWD, MODE, 6, 5, T>(s, a);
+}
+template <int MODE>
+static int launch_bwd(Solver* s, long long* trace = nullptr) {
+    return mixed(s) ? launch_bwd_t<MODE, float>(s, trace) : launch_bwd_t<MODE, double>(s, trace);
 }
 
 static long skew_n(Solver* s) { return (long)s->sk.nstrips * s->sk.strip_sz; }
@@ -923,8 +989,13 @@ static int launch_precon(Solver* s, long long* trace) {
     a.out[0] = s->precon;
     a.out[1] = s->cx;
     a.out[2] = s->cy;
+    if (mixed(s)) {
+        a.out[3] = s->pc32;
+        a.out[4] = s->cx32;
+        a.out[5] = s->cy32;
+    }
     a.trace = trace;
-    return launch_stream<SW_PRECON, 0, 3, 3>(s, a);
+    return launch_stream<SW_PRECON, 0, 3, 3, double>(s, a);
 }
 
 int k_build_precon(Solver* s) {
@@ -1080,8 +1151,6 @@ int k_project(Solver* s, int limit, ifl_solve_stats* stats) {
         stats->status = h.status;
         stats->iterations = h.printed_iter;
         stats->max_error = h.max_err;
-        stats->outer = 0;
-        stats->inner_total = 0;
     }
     if (s->cfg.verbose) {
         if (h.status == IFL_SOLVE_CONVERGED)

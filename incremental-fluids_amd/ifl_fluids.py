@@ -47,8 +47,7 @@ class Config(ctypes.Structure):
         ("density", ctypes.c_double), ("density_soot", ctypes.c_double), ("diffusion", ctypes.c_double),
         ("tau", ctypes.c_double), ("sigma", ctypes.c_double), ("tol", ctypes.c_double),
         ("t_amb", ctypes.c_double), ("gravity", ctypes.c_double), ("flip_alpha", ctypes.c_double),
-        ("mixed_inner_rtol", ctypes.c_double), ("mixed_max_outer", ctypes.c_int32),
-        ("reserved", ctypes.c_int32 * 7),
+        ("reserved", ctypes.c_int32 * 10),
     ]
 
 
@@ -59,8 +58,7 @@ class BodyC(ctypes.Structure):
 
 
 class SolveStats(ctypes.Structure):
-    _fields_ = [("status", ctypes.c_int32), ("iterations", ctypes.c_int32), ("max_error", ctypes.c_double),
-                ("outer", ctypes.c_int32), ("inner_total", ctypes.c_int32)]
+    _fields_ = [("status", ctypes.c_int32), ("iterations", ctypes.c_int32), ("max_error", ctypes.c_double)]
 
 
 class StepStats(ctypes.Structure):
