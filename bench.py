@@ -39,10 +39,11 @@ TRAFFIC_FILE = os.path.join(ROOT, "profiles", "traffic.json")
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--steps", type=int, default=4)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--size", type=int, default=2048)
-    ap.add_argument("--form", choices=["fp64", "mixed"], default="fp64")
+    ap.add_argument("--form", choices=["fp64", "mixed"], default=None,
+                    help="time only this form (default: FP64 as the value, the mixed form beside it)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-iters", type=int, default=30)
     ap.add_argument("--calibrate-copy", action="store_true",
@@ -124,51 +125,30 @@ def measured_traffic(kernel):
         return None
 
 
-def main():
-    global cpu_iters_global
-    args = parse()
-    if args.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 "
-                                   "(the multi-GPU strip decomposition is a design note only)"}))
-        sys.exit(2)
-    cpu_iters_global = args.cpu_iters
-    if args.calibrate_copy:
-        calibrate_copy()
-        return
+def run_form(form, n, steps, warmup):
+    """Creates the solver in `form`, runs `warmup` untimed steps, then times
+    `steps` whole update() calls (inputs resident in HBM) with per-kernel
+    HIP-event profiling on."""
     import ifl_fluids as F
     import ifl_scenes as S
-
-    n = args.size
     scene = S.config3_scene()
-    prec = F.PREC_MIXED if args.form == "mixed" else F.PREC_FP64
+    prec = F.PREC_MIXED if form == "mixed" else F.PREC_FP64
     objs = [F.SolidBox(*b[1:]) if b[0] == S.BOX else F.SolidSphere(b[1], b[2], b[3], *b[5:]) for b in scene["bodies"]]
     solver = F.FluidSolver(7, n, n, scene["params"], objs, precision=prec)
-
-    iters = []
-
-    def on_step(k, st):
-        iters.append((st.heat.iterations, st.pressure.iterations, st.heat.status, st.pressure.status))
-
-    S.drive(solver, scene, args.warmup)
+    S.drive(solver, scene, warmup)
     solver.synchronize()
-    iters.clear()
     solver.profile(True)
     solver.profile_reset()
     solver.synchronize()
+    iters = []
     t0 = time.perf_counter()
-    steps_done = [0]
-
-    def timed_step(k, st):
-        on_step(k, st)
-        steps_done[0] += 1
-
     # the scene's body motion cadence continues from the warmup
-    for k in range(args.steps):
+    for k in range(steps):
         for inf in scene["inflows"]:
             solver.add_inflow(*inf)
         st = solver.update(scene["dt"])
-        timed_step(k, st)
-        if (args.warmup + k + 1) % scene["substeps"] == 0:
+        iters.append((st.heat.iterations, st.pressure.iterations, st.heat.status, st.pressure.status))
+        if (warmup + k + 1) % scene["substeps"] == 0:
             solver.bodies_update(scene["dt"])
     solver.synchronize()
     wall = time.perf_counter() - t0
@@ -183,46 +163,73 @@ def main():
         bytes_launch = KERNEL_BYTES_PER_CELL[name] * n * n
         kern[name] = {"launches": p["launches"], "total_ms": p["ms"], "avg_us": 1e3 * avg_ms,
                       "bytes_per_launch": bytes_launch, "achieved_GBps": bytes_launch / (avg_ms * 1e-3) / 1e9}
-    dom = max(kern, key=lambda k: kern[k]["total_ms"]) if kern else None
-
-    steps_per_s = args.steps / wall
+    solver.close()
     pcg_iters = prof["iterations"]
     bpc = PCG_BYTES_PER_CELL_ITER[7]
-    pcg_gbs = bpc * prof["cell_iterations"] / (prof["ms"] * 1e-3) / 1e9 if prof["ms"] > 0 else 0.0
-    mean_iters = pcg_iters / max(1, args.steps)
+    return {
+        "scene": scene, "wall": wall, "steps_per_s": steps / wall, "ms_per_step": 1e3 * wall / steps,
+        "mean_iters": pcg_iters / max(1, steps), "solves": [list(x) for x in iters], "kernels": kern,
+        "pcg": {"launches": prof["launches"], "ms_total": prof["ms"], "iterations": pcg_iters,
+                "ms_per_iteration": prof["ms"] / max(1, pcg_iters), "bytes_per_cell_iteration": bpc,
+                "achieved_GBps": bpc * prof["cell_iterations"] / (prof["ms"] * 1e-3) / 1e9 if prof["ms"] > 0 else 0.0},
+    }
 
+
+def main():
+    global cpu_iters_global
+    args = parse()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 "
+                                   "(the multi-GPU strip decomposition is a design note only)"}))
+        sys.exit(2)
+    cpu_iters_global = args.cpu_iters
+    if args.calibrate_copy:
+        calibrate_copy()
+        return
+    n = args.size
+    # the headline value is the form asked for (FP64 by default); without
+    # --form the mixed form is run too and reported beside it
+    main_form = args.form or "fp64"
+    forms = [main_form] if args.form else ["fp64", "mixed"]
+    res = {f: run_form(f, n, args.steps, args.warmup) for f in forms}
+    r = res[main_form]
+    kern = r["kernels"]
+    dom = max(kern, key=lambda k: kern[k]["total_ms"]) if kern else None
     out = {
-        "metric": "solver steps/s at 2048^2 (solver 7, FP64 drop-in); PCG HBM GB/s",
-        "value": steps_per_s,
+        "metric": "solver steps/s at 2048^2 (solver 7; FP64 drop-in, mixed form beside it); PCG HBM GB/s",
+        "value": r["steps_per_s"],
         "unit": "steps/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": 1e3 * wall / args.steps,
+        "ms_per_step": r["ms_per_step"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64" if args.form == "fp64" else "f32+f64",
+        "dtype": "f64" if main_form == "fp64" else "f32+f64",
         "data": "synthetic",
         "config": {"workload": f"solver7-variable-density-{n}x{n}-box-2inflows-seeded",
-                   "grid": [n, n], "level": 7, "form": args.form, "dt": scene["dt"],
-                   "pcg_iterations_per_step": mean_iters,
-                   "solves": [list(x) for x in iters]},
-        "pcg": {"launches": prof["launches"], "ms_total": prof["ms"], "iterations": pcg_iters,
-                "ms_per_iteration": prof["ms"] / max(1, pcg_iters), "bytes_per_cell_iteration": bpc,
-                "achieved_GBps": pcg_gbs},
+                   "grid": [n, n], "level": 7, "form": main_form, "dt": r["scene"]["dt"],
+                   "pcg_iterations_per_step": r["mean_iters"], "solves": r["solves"]},
+        "pcg": r["pcg"],
         "kernels": kern,
     }
     if dom:
         k = kern[dom]
         out["roofline"] = {"bound": "hbm", "achieved": k["achieved_GBps"], "peak": PEAK_HBM_GBS, "unit": "GB/s",
-                           "frac": k["achieved_GBps"] / PEAK_HBM_GBS, "traffic": measured_traffic(dom),
+                           "frac": k["achieved_GBps"] / PEAK_HBM_GBS,
+                           "traffic": measured_traffic(dom) if main_form == "fp64" else None,
                            "kernel": f"{dom}: {KERNEL_BYTES_PER_CELL[dom]:.0f} B/cell x {n}x{n} cells per launch, "
                                      f"average over {k['launches']} launches (HIP events on the solver's stream)"}
-    solver.close()
+    if len(forms) > 1:
+        out["forms"] = {f: {"steps_per_s": res[f]["steps_per_s"], "ms_per_step": res[f]["ms_per_step"],
+                            "pcg_iterations_per_step": res[f]["mean_iters"],
+                            "ms_per_pcg_iteration": res[f]["pcg"]["ms_per_iteration"],
+                            "sweep_us": {k: v["avg_us"] for k, v in res[f]["kernels"].items()}} for f in forms}
+        out["forms"]["mixed_over_fp64"] = res["mixed"]["steps_per_s"] / res["fp64"]["steps_per_s"]
     if not args.no_cpu_baseline:
         try:
-            out["cpu_baseline"] = cpu_baseline(scene, n, mean_iters)
+            out["cpu_baseline"] = cpu_baseline(r["scene"], n, r["mean_iters"])
         except Exception as e:  # the baseline is reported, never the measured value
             out["cpu_baseline"] = {"error": str(e)}
     print(json.dumps(out))
