@@ -154,9 +154,21 @@ enum {
 int ifl_field_info(const ifl_solver* s, int field, size_t* count, int* elem_bytes);
 int ifl_upload(ifl_solver* s, int field, const void* host, size_t bytes);
 int ifl_download(ifl_solver* s, int field, void* host, size_t bytes);
-/* Level 8: the live particle count (_particleCount 8:701). */
+/* Level 8: the live particle count (_particleCount 8:701). Setting it hands
+ * the particle state to the caller: ifl_update then skips initialisation. */
 int ifl_get_particle_count(const ifl_solver* s, int* count);
 int ifl_set_particle_count(ifl_solver* s, int count);
+
+/* Level 8: the rest of the FLIP constructor, ParticleQuantities'
+ * initParticles (8:735-751, 4 jittered particles per cell from frand, those
+ * inside a body dropped) and FluidSolver's gridToParticles(1.0) (8:1314).
+ * Call once after ifl_set_bodies; ifl_update calls it if it has not run. */
+int ifl_init_particles(ifl_solver* s);
+
+/* frand()'s seed (8:38-46): one per process, 0xBA5EBA11 at load, advanced by
+ * every level-8 solver in creation and step order as in the reference. */
+uint32_t ifl_frand_state(void);
+void ifl_frand_set_state(uint32_t state);
 
 /* ---- single stages of update(), for teacher-forced parity tests ----
  * iarg / darg meaning per stage is given on each line. `result` receives the

@@ -146,6 +146,8 @@ class FluidSolver {
         _tAmb = cfg.t_amb;
         ifl_detail::check(ifl_create(&cfg, &_s), nullptr);
         push_bodies();
+        /* the FLIP constructor seeds its particles from the bodies (8:1308-1314) */
+        if (IFL_SOLVER_LEVEL == 8) ifl_detail::check(ifl_init_particles(_s), _s);
     }
     void push_bodies() {
         if (!_bodies) return;

@@ -3,6 +3,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <new>
 
 #include "ifl_solver.hpp"
@@ -14,6 +15,21 @@ struct ifl_solver {
 };
 
 namespace ifl {
+
+// frand() 8:38-46 keeps one static seed for the whole process; every level-8
+// solver draws from it in creation and step order, as in the reference.
+static std::mutex g_frand_mu;
+static uint32_t g_frand = 0xBA5EBA11u;  // 8:39
+
+uint32_t frand_state() {
+    std::lock_guard<std::mutex> lock(g_frand_mu);
+    return g_frand;
+}
+
+void frand_set_state(uint32_t state) {
+    std::lock_guard<std::mutex> lock(g_frand_mu);
+    g_frand = state;
+}
 
 static int ev_take(Solver* s, hipEvent_t* e) {
     if (!s->ev_free.empty()) {
@@ -167,6 +183,7 @@ void free_all(Solver* s) {
     F(s->posx); F(s->posy);
     for (int t = 0; t < 4; t++) F(s->prop[t]);
     F(s->weight); F(s->counts);
+    flip_free(s);
     if (s->st_host) (void)hipHostFree(s->st_host);
     if (s->ev0) (void)hipEventDestroy(s->ev0);
     if (s->ev1) (void)hipEventDestroy(s->ev1);
@@ -259,6 +276,19 @@ int init_solver(Solver* s) {
     IFL_HIP(hipMalloc((void**)&s->front_cnt, 2 * sizeof(int)));
     IFL_HIP(hipMemsetAsync(s->front_cnt, 0, 2 * sizeof(int), s->stream));
     IFL_HIP(hipMalloc((void**)&s->bodies_d, kMaxBodies * sizeof(Body)));
+    if (lv == 8) {
+        // ParticleQuantities 8:865-878: room for 12 particles per cell; the
+        // property arrays start at zero (8:896-897)
+        s->max_particles = W * H * 12;
+        size_t mp = (size_t)s->max_particles;
+        IFL_TRY(dalloc(s, &s->posx, mp));
+        IFL_TRY(dalloc(s, &s->posy, mp));
+        for (int t = 0; t < 4; t++) IFL_TRY(dalloc(s, &s->prop[t], mp));
+        IFL_TRY(dalloc(s, &s->weight, (size_t)(W + 1) * (H + 1)));
+        IFL_HIP(hipMalloc((void**)&s->counts, (size_t)W * H * sizeof(int)));
+        IFL_HIP(hipMemsetAsync(s->counts, 0, (size_t)W * H * sizeof(int), s->stream));
+        IFL_TRY(flip_alloc(s));
+    }
     IFL_TRY(k_update_dflag(s));
     IFL_HIP(hipStreamSynchronize(s->stream));
     return IFL_OK;
@@ -285,6 +315,7 @@ int read_border_flag(Solver* s) {
 }
 
 int extrapolate(Solver* s, int qi) {
+    if (s->level == 8) return k_extrapolate_flip(s, qi);
     IFL_TRY(k_extrapolate(s, qi));
     return IFL_OK;
 }
@@ -295,8 +326,8 @@ int fill_solid(Solver* s, int qi) {
     return IFL_OK;
 }
 
-// FluidSolver::update for levels 3..7 (3:433-447, 4:869-895, 5:927-953,
-// 6:966-1008, 7:995-1034)
+// FluidSolver::update for levels 3..8 (3:433-447, 4:869-895, 5:927-953,
+// 6:966-1008, 7:995-1034, 8:1350-1413)
 int update_grid(Solver* s, double dt, ifl_step_stats* stats) {
     int lv = s->level;
     ifl_solve_stats heat = {IFL_SOLVE_NOT_RUN, 0, 0.0}, press = heat;
@@ -312,10 +343,22 @@ int update_grid(Solver* s, double dt, ifl_step_stats* stats) {
         flip_q(s, IFL_Q_U);
         flip_q(s, IFL_Q_V);
     } else {
+        if (lv == 8 && !s->particles_ready) IFL_TRY(flip_init_particles(s));
         IFL_TRY(fill_solid(s, IFL_Q_D));
         if (lv >= 6) IFL_TRY(fill_solid(s, IFL_Q_T));
         IFL_TRY(fill_solid(s, IFL_Q_U));
         IFL_TRY(fill_solid(s, IFL_Q_V));
+        if (lv == 8) {
+            // 8:1357-1368: particles to grid, the pre-update copies, and the
+            // inflow the FLIP solver applies itself
+            IFL_TRY(k_particles_to_grid(s));
+            IFL_TRY(k_update_dflag(s));
+            if (s->cfg.verbose) printf("Particle count: %d\n", s->particle_count);
+            for (int qi = 0; qi < 4; qi++) IFL_TRY(k_copy_old(s, qi));
+            const double x = 0.45, y = 0.2, w = 0.2, h = 0.05;
+            const double vals[4] = {1.0, s->cfg.t_amb, 0.0, 0.0};
+            IFL_TRY(k_add_inflow(s, x, y, x + w, y + h, vals));
+        }
         if (lv >= 6) {
             IFL_TRY(k_copy_t_to_r(s));
             IFL_TRY(k_build_heat(s, dt));
@@ -335,14 +378,24 @@ int update_grid(Solver* s, double dt, ifl_step_stats* stats) {
         IFL_TRY(extrapolate(s, IFL_Q_D));
         IFL_TRY(extrapolate(s, IFL_Q_U));
         IFL_TRY(extrapolate(s, IFL_Q_V));
-        IFL_TRY(read_border_flag(s));
+        if (lv < 8) IFL_TRY(read_border_flag(s));
         IFL_TRY(k_set_boundary(s));
-        int qm = (1 << IFL_Q_D) | (1 << IFL_Q_U) | (1 << IFL_Q_V) | (lv >= 6 ? (1 << IFL_Q_T) : 0);
-        IFL_TRY(k_advect(s, qm, dt));
-        flip_q(s, IFL_Q_D);
-        if (lv >= 6) flip_q(s, IFL_Q_T);
-        flip_q(s, IFL_Q_U);
-        flip_q(s, IFL_Q_V);
+        if (lv == 8) {
+            // 8:1393-1412: FLIP update of the particles, then their advection
+            const double alpha = s->cfg.flip_alpha;
+            for (int qi = 0; qi < 4; qi++) IFL_TRY(k_diff(s, qi, alpha, 0));
+            IFL_TRY(k_grid_to_particles(s, alpha));
+            for (int qi = 0; qi < 4; qi++) IFL_TRY(k_diff(s, qi, alpha, 1));
+            IFL_TRY(k_particle_advect(s, dt));
+            IFL_TRY(flip_check_overflow(s));
+        } else {
+            int qm = (1 << IFL_Q_D) | (1 << IFL_Q_U) | (1 << IFL_Q_V) | (lv >= 6 ? (1 << IFL_Q_T) : 0);
+            IFL_TRY(k_advect(s, qm, dt));
+            flip_q(s, IFL_Q_D);
+            if (lv >= 6) flip_q(s, IFL_Q_T);
+            flip_q(s, IFL_Q_U);
+            flip_q(s, IFL_Q_V);
+        }
     }
     IFL_HIP(hipStreamSynchronize(s->stream));
     IFL_TRY(prof_collect(s, nullptr));
@@ -396,9 +449,9 @@ int ifl_create(const ifl_config* cfg, ifl_solver** out) {
         g_err = "ifl_create: level must be 3..8 and w, h >= 3";
         return IFL_ERR_ARG;
     }
-    if (cfg->level == 8) {
-        g_err = "ifl_create: level 8 (FLIP) is not available in this build";
-        return IFL_ERR_UNSUPPORTED;
+    if (cfg->level == 8 && (long)cfg->w * cfg->h * 12 > 0x7FFFFFFFL) {
+        g_err = "ifl_create: level 8 keeps 12 particles per cell; w*h*12 must fit in an int";
+        return IFL_ERR_ARG;
     }
     if (cfg->precision != IFL_PREC_FP64 && cfg->precision != IFL_PREC_MIXED) {
         g_err = "ifl_create: precision must be IFL_PREC_FP64 or IFL_PREC_MIXED";
@@ -550,8 +603,24 @@ int ifl_set_particle_count(ifl_solver* h, int count) {
     if (s->level != 8) return IFL_ERR_UNSUPPORTED;
     if (count < 0 || count > s->max_particles) return set_err(s, IFL_ERR_ARG, "particle count out of range");
     s->particle_count = count;
+    s->particles_ready = true;  // the caller now owns the particle state
     return IFL_OK;
 }
+
+int ifl_init_particles(ifl_solver* h) {
+    if (!h) return IFL_ERR_ARG;
+    Solver* s = &h->s;
+    if (s->level != 8) return IFL_ERR_UNSUPPORTED;
+    if (s->particles_ready) return set_err(s, IFL_ERR_ARG, "ifl_init_particles: particles already initialised");
+    IFL_HIP(hipSetDevice(s->device));
+    IFL_TRY(flip_init_particles(s));
+    IFL_HIP(hipStreamSynchronize(s->stream));
+    return IFL_OK;
+}
+
+uint32_t ifl_frand_state(void) { return frand_state(); }
+
+void ifl_frand_set_state(uint32_t state) { frand_set_state(state); }
 
 int ifl_stage(ifl_solver* h, int stage, int iarg, double darg, ifl_solve_stats* stats, double* result) {
     if (!h) return IFL_ERR_ARG;
@@ -630,10 +699,40 @@ int ifl_stage(ifl_solver* h, int stage, int iarg, double darg, ifl_solve_stats* 
         if (lv < 6) return IFL_ERR_UNSUPPORTED;
         rc = k_copy_p_to_t(s);
         break;
+    case IFL_STAGE_FROM_PARTICLES:
+    case IFL_STAGE_COPY_OLD:
+    case IFL_STAGE_DIFF:
+    case IFL_STAGE_UNDIFF:
+        if (lv != 8) return IFL_ERR_UNSUPPORTED;
+        if (!qok) return IFL_ERR_ARG;
+        rc = stage == IFL_STAGE_FROM_PARTICLES ? k_from_particles(s, iarg, false)
+             : stage == IFL_STAGE_COPY_OLD     ? k_copy_old(s, iarg)
+                                               : k_diff(s, iarg, darg, stage == IFL_STAGE_UNDIFF);
+        break;
+    case IFL_STAGE_G2P:
+    case IFL_STAGE_P2G:
+    case IFL_STAGE_PARTICLE_ADVECT:
+    case IFL_STAGE_COUNT_PARTICLES:
+    case IFL_STAGE_PRUNE:
+    case IFL_STAGE_SEED:
+        if (lv != 8) return IFL_ERR_UNSUPPORTED;
+        rc = stage == IFL_STAGE_G2P               ? k_grid_to_particles(s, darg)
+             : stage == IFL_STAGE_P2G             ? k_particles_to_grid(s)
+             : stage == IFL_STAGE_PARTICLE_ADVECT ? k_particle_advect(s, darg)
+             : stage == IFL_STAGE_COUNT_PARTICLES ? k_count_particles(s)
+             : stage == IFL_STAGE_PRUNE           ? k_prune(s)
+                                                  : k_seed(s);
+        break;
     default:
         return set_err(s, IFL_ERR_UNSUPPORTED, "ifl_stage: unknown stage or not at this level");
     }
     if (rc != IFL_OK) return rc;
+    if (lv == 8) {
+        // EMPTY marks on the d grid change the cells the pressure solve visits
+        if (stage == IFL_STAGE_FROM_PARTICLES || stage == IFL_STAGE_EXTRAPOLATE || stage == IFL_STAGE_P2G)
+            IFL_TRY(k_update_dflag(s));
+        if (stage == IFL_STAGE_EXTRAPOLATE || stage == IFL_STAGE_P2G) IFL_TRY(flip_check_overflow(s));
+    }
     IFL_HIP(hipStreamSynchronize(s->stream));
     return IFL_OK;
 }

@@ -36,6 +36,25 @@ IFL_DEV double cubic_pulse(double x) {
 // to 0 so a corrupted input can never index outside an array.
 IFL_DEV int trunc_idx(double x) { return (x == x) ? (int)x : 0; }
 
+// frand's generator 8:38-46: seed = (seed*1103515245 + 12345) & 0x7FFFFFFF.
+// Masking each step is arithmetic modulo 2^31, so n steps are one affine map
+// s -> A*s + C (mod 2^31), built by repeated squaring (jump-ahead).
+__host__ __device__ inline uint32_t lcg_next(uint32_t s) { return (s * 1103515245u + 12345u) & 0x7FFFFFFFu; }
+__host__ __device__ inline uint32_t lcg_jump(uint32_t s, unsigned long long n) {
+    if (n == 0) return s;
+    uint32_t A = 1u, C = 0u, a = 1103515245u, c = 12345u;
+    while (n) {
+        if (n & 1) {
+            C = a * C + c;
+            A = a * A;
+        }
+        c = a * c + c;
+        a = a * a;
+        n >>= 1;
+    }
+    return (A * s + C) & 0x7FFFFFFFu;
+}
+
 // ---------------------------------------------------------------- bodies
 // Device copy of a SolidBody (7:109-257) with host-evaluated cos/sin.
 struct Body {

@@ -86,6 +86,18 @@ struct Solver {
     double *posx = nullptr, *posy = nullptr, *prop[4] = {nullptr, nullptr, nullptr, nullptr};
     double* weight = nullptr;
     int* counts = nullptr;
+    bool particles_ready = false;  // initParticles has run
+    // particle work buffers: sort keys / indices and flags (max_particles
+    // each), per-cell integer scratch ((w+1)(h+1)+1 each), device scalars
+    uint32_t *pkey = nullptr, *pkey2 = nullptr;
+    int *pidx = nullptr, *pidx2 = nullptr;
+    uint8_t* pflag = nullptr;
+    int *cbuf0 = nullptr, *cbuf1 = nullptr, *cbuf2 = nullptr, *cbuf3 = nullptr;
+    uint8_t* cflag = nullptr;
+    void* flip_sc = nullptr;    // device FlipScalars
+    void* flip_host = nullptr;  // pinned mirror
+    void* cub_tmp = nullptr;    // hipcub temporary storage
+    size_t cub_bytes = 0;
 
     // timing
     bool profiling = false;
@@ -164,16 +176,23 @@ constexpr int kTraceStamps = 6;
 int k_trace_sweep(Solver* s, int kind, long long* dev_trace);
 
 // ---- launchers (k_flip.hip) ----
+int flip_alloc(Solver* s);
+void flip_free(Solver* s);
 int flip_init_particles(Solver* s);
-int k_from_particles(Solver* s, int qi);
+int k_from_particles(Solver* s, int qi, bool with_t);
+int k_extrapolate_flip(Solver* s, int qi);
 int k_grid_to_particles(Solver* s, double alpha);
 int k_particles_to_grid(Solver* s);
 int k_particle_advect(Solver* s, double dt);
 int k_copy_old(Solver* s, int qi);
-int k_diff(Solver* s, int qi, double alpha, double sign);
+int k_diff(Solver* s, int qi, double alpha, int undo);
 int k_count_particles(Solver* s);
 int k_prune(Solver* s);
 int k_seed(Solver* s);
+int flip_check_overflow(Solver* s);
+// frand()'s process-wide state (8:39), shared by every level-8 solver
+uint32_t frand_state();
+void frand_set_state(uint32_t state);
 
 inline int grid_blocks(long n, int threads) { return (int)((n + threads - 1) / threads); }
 

@@ -97,6 +97,10 @@ def _load():
     lib.ifl_synchronize.argtypes = [vp]
     lib.ifl_get_particle_count.argtypes = [vp, ctypes.POINTER(ctypes.c_int)]
     lib.ifl_set_particle_count.argtypes = [vp, ctypes.c_int]
+    lib.ifl_init_particles.argtypes = [vp]
+    lib.ifl_frand_state.restype = ctypes.c_uint32
+    lib.ifl_frand_state.argtypes = []
+    lib.ifl_frand_set_state.argtypes = [ctypes.c_uint32]
     return lib
 
 
@@ -167,6 +171,9 @@ class FluidSolver:
         self._check(lib.ifl_create(ctypes.byref(cfg), ctypes.byref(self._h)), None)
         self.last_stats = StepStats()
         self._push_bodies()
+        if level == 8:
+            # the rest of the FLIP constructor (8:1308-1314)
+            self._check(lib.ifl_init_particles(self._h))
 
     def _check(self, rc, handle=True):
         if rc != 0:
@@ -251,6 +258,24 @@ class FluidSolver:
 
     def synchronize(self):
         self._check(lib.ifl_synchronize(self._h))
+
+    def particle_count(self, value=None):
+        """Level 8: the live particle count (_particleCount 8:701)."""
+        if value is not None:
+            self._check(lib.ifl_set_particle_count(self._h, value))
+            return value
+        n = ctypes.c_int()
+        self._check(lib.ifl_get_particle_count(self._h, ctypes.byref(n)))
+        return n.value
+
+
+def frand_state():
+    """frand()'s process-wide seed (8:38-46)."""
+    return lib.ifl_frand_state()
+
+
+def frand_set_state(state):
+    lib.ifl_frand_set_state(state & 0xFFFFFFFF)
 
 
 def reference_log_line(level, stats):

@@ -1,4 +1,4 @@
-// A reference-style main() (after 4:919-965 and 7:1084-1129) built against
+// A reference-style main() (after 4:919-965, 7:1084-1129, 8:1469-1513) built against
 // the source-level drop-in header instead of Fluid.cpp. Usage:
 //   shim_main <w> <h> <steps> <density-out.bin>
 // Prints what the reference prints per update() and writes the final density
@@ -17,7 +17,7 @@ int main(int argc, char** argv) {
         return 2;
     }
     const int w = atoi(argv[1]), h = atoi(argv[2]), steps = atoi(argv[3]);
-    const double dt = 0.005;
+    const double dt = IFL_SOLVER_LEVEL == 8 ? 0.0025 : 0.005;  // 8:1477
     std::vector<const SolidBody*> bodies;
 #if IFL_SOLVER_LEVEL >= 4
     SolidBox box(0.5, 0.6, 0.7, 0.1, M_PI * 0.25, 0.0, 0.0, 0.0);
@@ -27,13 +27,15 @@ int main(int argc, char** argv) {
     FluidSolver solver(w, h, 0.1);
 #elif IFL_SOLVER_LEVEL <= 5
     FluidSolver solver(w, h, 0.1, bodies);
-#else
+#elif IFL_SOLVER_LEVEL <= 7
     FluidSolver solver(w, h, 0.1, 1.0, 0.01, bodies);
+#else
+    FluidSolver solver(w, h, 0.1, 0.25, 0.01, bodies);  // 8:1490; the inflow is inside update() (8:1368)
 #endif
     for (int k = 0; k < steps; k++) {
 #if IFL_SOLVER_LEVEL <= 5
         solver.addInflow(0.45, 0.2, 0.15, 0.03, 1.0, 0.0, 3.0);
-#else
+#elif IFL_SOLVER_LEVEL <= 7
         solver.addInflow(0.45, 0.2, 0.1, 0.05, 1.0, solver.ambientT(), 0.0, 0.0);
 #endif
         solver.update(dt);

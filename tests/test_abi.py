@@ -13,13 +13,14 @@ LIB = os.path.join(ROOT, "incremental-fluids_amd", "libifl.so")
 
 def declared():
     text = open(HEADER).read()
-    return sorted(set(re.findall(r"^\s*(?:int|const char\*)\s+(ifl_\w+)\s*\(", text, re.M)))
+    return sorted(set(re.findall(r"^\s*(?:int|void|uint32_t|const char\*)\s+(ifl_\w+)\s*\(", text, re.M)))
 
 
 def test_header_declares_entry_points():
     names = declared()
     for n in ("ifl_create", "ifl_destroy", "ifl_update", "ifl_add_inflow", "ifl_set_bodies",
-              "ifl_upload", "ifl_download", "ifl_stage", "ifl_profile_read"):
+              "ifl_upload", "ifl_download", "ifl_stage", "ifl_profile_read", "ifl_init_particles",
+              "ifl_frand_state", "ifl_frand_set_state"):
         assert n in names
 
 
@@ -58,3 +59,16 @@ def test_no_cpu_fallback_without_a_device():
         return
     s.close()
     pytest.skip("a GPU is present")
+
+
+def test_frand_state_is_process_wide_host_state():
+    """frand()'s seed (8:39) starts at 0xBA5EBA11 and is read and set on the
+    host, without a device."""
+    import ifl_fluids as F
+    old = F.frand_state()
+    try:
+        F.frand_set_state(0x12345678)
+        assert F.frand_state() == 0x12345678
+    finally:
+        F.frand_set_state(old)
+    assert F.frand_state() == old

@@ -29,17 +29,17 @@ def build_shim(level, outdir):
 
 
 @pytest.mark.skipif(not os.path.exists(LIB), reason="libifl.so not built")
-@pytest.mark.parametrize("level", [3, 4, 5, 6, 7])
+@pytest.mark.parametrize("level", [3, 4, 5, 6, 7, 8])
 def test_shim_compiles(level, tmp_path):
     assert os.path.exists(build_shim(level, str(tmp_path)))
 
 
 SHIM_SCENE = {3: ol.REF_SCENES[3], 4: ol.REF_SCENES[4],
-              6: dict(ol.REF_SCENES[7]), 7: ol.REF_SCENES[7]}
+              6: dict(ol.REF_SCENES[7]), 7: ol.REF_SCENES[7], 8: ol.REF_SCENES[8]}
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("level", [3, 4, 6, 7])
+@pytest.mark.parametrize("level", [3, 4, 6, 7, 8])
 def test_shim_matches_oracle(level, tmp_path):
     w, h, steps = 64, 48, 6
     exe = build_shim(level, str(tmp_path))
@@ -52,7 +52,8 @@ def test_shim_matches_oracle(level, tmp_path):
     sc = SHIM_SCENE[level]
     orc = ol.OracleSolver(level, w, h, sc["params"], sc["bodies"], fresh=True)
     for _ in range(steps):
-        orc.add_inflow(*sc["inflow"])
+        if sc["inflow"] is not None:
+            orc.add_inflow(*sc["inflow"])
         orc.update(sc["dt"])
     assert run.stdout == orc.log()
     d = np.fromfile(out, dtype=np.float64)
