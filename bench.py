@@ -41,7 +41,9 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=4)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--size", type=int, default=2048)
+    ap.add_argument("--size", type=int, default=None, help="grid side (default 2048; 1024 for --workload flip)")
+    ap.add_argument("--workload", choices=["solver7", "flip"], default="solver7",
+                    help="solver7: the headline 2048^2 variable-density step; flip: solver 8 at 1024^2")
     ap.add_argument("--form", choices=["fp64", "mixed"], default=None,
                     help="time only this form (default: FP64 as the value, the mixed form beside it)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -175,6 +177,108 @@ def run_form(form, n, steps, warmup):
     }
 
 
+# Algorithmic bytes of the particle kernels: per particle (P) and per grid
+# node (N) of the grid the launch writes or samples, each array once.
+#   P2G gather, d with t: posx, posy, prop d, prop t, sorted index (36 B/P);
+#     group start, src d, src t, weight, two cell bytes (30 B/N)
+#   P2G gather, u or v: 28 B/P; 21 B/N
+#   G2P: posx, posy read, four properties read and written (80 B/P); the four
+#     grids read (32 B/N)
+#   particle advect: posx, posy read and written (32 B/P); u, v read (16 B/N)
+def flip_kernel_bytes(n, p_before, p_after):
+    nd, nu, nv = n * n, (n + 1) * n, n * (n + 1)
+    return {
+        "p2g": (36 * p_before + 30 * nd) + (28 * p_before + 21 * nu) + (28 * p_before + 21 * nv),
+        "g2p": 80 * p_after + 32 * nd,
+        "particle_advect": 32 * p_after + 16 * nd,
+    }
+
+
+def cpu_baseline_flip(scene, n):
+    """One update() of the reference's solver 8 (or the C restatement) at the
+    same size on one host core, from its own constructor."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_lib as ol
+    kind = "reference" if ol.ref_available(8) else "port"
+    S = ol.RefSolver if kind == "reference" else ol.OracleSolver
+    s = S(8, n, n, scene["params"], scene["bodies"], fresh=True)
+    t0 = time.perf_counter()
+    s.update(scene["dt"])
+    step_s = time.perf_counter() - t0
+    return dict(value=1.0 / step_s, unit="steps/s", cores=1, kind=kind,
+                sample=f"one update() of solver 8 at {n}x{n} from a fresh solver on one host core ({step_s:.1f} s)",
+                seconds_per_step=step_s)
+
+
+def run_flip(args):
+    """Solver 8 (FLIP) on the reference's own scene: steps/s, the PCG
+    kernels and the particle kernels (HIP events on the solver's stream)."""
+    import ifl_fluids as F
+    import ifl_scenes as S
+    n = args.size or 1024
+    scene = S.config5_scene()
+    F.frand_set_state(0xBA5EBA11)
+    objs = [F.SolidBox(*b[1:]) for b in scene["bodies"]]
+    solver = F.FluidSolver(8, n, n, scene["params"], objs)
+    for _ in range(args.warmup):
+        solver.update(scene["dt"])
+    solver.synchronize()
+    solver.profile(True)
+    solver.profile_reset()
+    byt = {"p2g": 0, "g2p": 0, "particle_advect": 0}
+    solves, counts = [], []
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        p0 = solver.particle_count()
+        st = solver.update(scene["dt"])
+        for k, v in flip_kernel_bytes(n, p0, st.particle_count).items():
+            byt[k] += v
+        solves.append([st.heat.iterations, st.pressure.iterations])
+        counts.append(st.particle_count)
+    solver.synchronize()
+    wall = time.perf_counter() - t0
+    kern = {}
+    for name, kid in (("sweep_fwd", F.KERNEL_SWEEP_FWD), ("sweep_bwd", F.KERNEL_SWEEP_BWD),
+                      ("matvec", F.KERNEL_MATVEC)):
+        p = solver.profile_read(kid)
+        if p["launches"]:
+            b = KERNEL_BYTES_PER_CELL[name] * n * n
+            avg = p["ms"] / p["launches"]
+            kern[name] = {"launches": p["launches"], "total_ms": p["ms"], "avg_us": 1e3 * avg,
+                          "bytes_per_launch": b, "achieved_GBps": b / (avg * 1e-3) / 1e9}
+    for name, kid in (("p2g", F.KERNEL_P2G), ("g2p", F.KERNEL_G2P),
+                      ("particle_advect", F.KERNEL_PARTICLE_ADVECT)):
+        p = solver.profile_read(kid)
+        if p["launches"]:
+            kern[name] = {"launches": p["launches"], "total_ms": p["ms"], "avg_us": 1e3 * p["ms"] / p["launches"],
+                          "bytes_per_launch": byt[name] / p["launches"],
+                          "achieved_GBps": byt[name] / (p["ms"] * 1e-3) / 1e9}
+    prof = solver.profile_read(F.KERNEL_PCG)
+    solver.close()
+    dom = max(("sweep_fwd", "sweep_bwd", "matvec"), key=lambda k: kern.get(k, {"total_ms": -1})["total_ms"])
+    k = kern[dom]
+    out = {
+        "metric": "solver 8 (FLIP) steps/s at 1024^2, FP64 drop-in",
+        "value": args.steps / wall, "unit": "steps/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": 1e3 * wall / args.steps, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64", "data": "synthetic",
+        "config": {"workload": f"solver8-flip-{n}x{n}-reference-scene", "grid": [n, n], "level": 8,
+                   "dt": scene["dt"], "particles": counts, "solves": solves},
+        "pcg": {"iterations": prof["iterations"], "ms_total": prof["ms"],
+                "ms_per_iteration": prof["ms"] / max(1, prof["iterations"])},
+        "kernels": kern,
+        "roofline": {"bound": "hbm", "achieved": k["achieved_GBps"], "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                     "frac": k["achieved_GBps"] / PEAK_HBM_GBS, "traffic": None,
+                     "kernel": f"{dom}: {KERNEL_BYTES_PER_CELL[dom]:.0f} B/cell x {n}x{n} cells per launch"},
+    }
+    if not args.no_cpu_baseline:
+        try:
+            out["cpu_baseline"] = cpu_baseline_flip(scene, n)
+        except Exception as e:
+            out["cpu_baseline"] = {"error": str(e)}
+    print(json.dumps(out))
+
+
 def main():
     global cpu_iters_global
     args = parse()
@@ -186,7 +290,10 @@ def main():
     if args.calibrate_copy:
         calibrate_copy()
         return
-    n = args.size
+    if args.workload == "flip":
+        run_flip(args)
+        return
+    n = args.size or 2048
     # the headline value is the form asked for (FP64 by default); without
     # --form the mixed form is run too and reported beside it
     main_form = args.form or "fp64"

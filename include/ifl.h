@@ -216,7 +216,10 @@ enum {
     IFL_KERNEL_SWEEP_FWD = 3,  /* r -= alpha*q, max|r| and the forward MIC(0) sweep, one launch per PCG iteration */
     IFL_KERNEL_SWEEP_BWD = 4,  /* backward MIC(0) sweep and the z.r partials, one launch per PCG iteration */
     IFL_KERNEL_MATVEC = 5,     /* q = A s and the q.s partials, one launch per PCG iteration */
-    IFL_KERNEL_COUNT = 6
+    IFL_KERNEL_P2G = 6,        /* level 8: fromParticles gather of one grid (d with t, u, v) */
+    IFL_KERNEL_G2P = 7,        /* level 8: gridToParticles of the four properties */
+    IFL_KERNEL_PARTICLE_ADVECT = 8, /* level 8: ParticleQuantities::advect */
+    IFL_KERNEL_COUNT = 9
 };
 int ifl_profile_enable(ifl_solver* s, int on);
 /* launches, summed milliseconds, summed PCG iterations (project() only) and

@@ -823,8 +823,10 @@ int k_grid_to_particles(Solver* s, double alpha) {
     a.posy = s->posy;
     a.n = n;
     a.alpha = alpha;
+    IFL_TRY(prof_begin(s, IFL_KERNEL_G2P, -1));
     g2p_kernel<<<grid_blocks(n, 256), 256, 0, s->stream>>>(a);
     IFL_LAUNCH_CHECK();
+    IFL_TRY(prof_end(s));
     return IFL_OK;
 }
 
@@ -943,8 +945,10 @@ int k_from_particles(Solver* s, int qi, bool with_t) {
     a.start = start;
     a.order = s->pidx2;
     a.weight = s->weight;
+    IFL_TRY(prof_begin(s, IFL_KERNEL_P2G, -1));
     p2g_gather_kernel<<<grid_blocks(cells, 256), 256, 0, s->stream>>>(a);
     IFL_LAUNCH_CHECK();
+    IFL_TRY(prof_end(s));
     return IFL_OK;
 }
 
@@ -998,8 +1002,10 @@ int k_particle_advect(Solver* s, double dt) {
     a.dt = dt;
     a.bodies = s->bodies_d;
     a.nb = s->nbodies;
+    IFL_TRY(prof_begin(s, IFL_KERNEL_PARTICLE_ADVECT, -1));
     particle_advect_kernel<<<grid_blocks(n, 256), 256, 0, s->stream>>>(a);
     IFL_LAUNCH_CHECK();
+    IFL_TRY(prof_end(s));
     return IFL_OK;
 }
 

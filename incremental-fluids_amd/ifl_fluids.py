@@ -29,6 +29,7 @@ PREC_FP64, PREC_MIXED = 0, 1
 DOT_TREE, DOT_SEQUENTIAL = 0, 1
 SOLVE_CONVERGED, SOLVE_INITIAL_SMALL, SOLVE_BUDGET, SOLVE_NOT_RUN = 0, 1, 2, 3
 KERNEL_PCG, KERNEL_PRECON, KERNEL_ADVECT, KERNEL_SWEEP_FWD, KERNEL_SWEEP_BWD, KERNEL_MATVEC = 0, 1, 2, 3, 4, 5
+KERNEL_P2G, KERNEL_G2P, KERNEL_PARTICLE_ADVECT = 6, 7, 8
 
 STAGE = dict(
     FILL_SOLID=1, EXTRAPOLATE=2, ADVECT=3, FLIP=4, BUILD_RHS=5, BUILD_PRESSURE=6,

@@ -41,6 +41,15 @@ def smoke_scene():
                 inflows=[(0.45, 0.2, 0.1, 0.05, 1.0, 294.0, 0.0, 0.0)], substeps=4)
 
 
+def config5_scene():
+    """Solver 8 (FLIP), the reference's own scene (8:1469-1513): the box of
+    8:1484, rho_air 0.1, rho_soot 0.25, diffusion 0.01, dt 0.0025; the inflow
+    is applied inside update() (8:1368). Particles start at 4 per cell from
+    frand() (8:735-751)."""
+    return dict(level=8, params=(0.1, 0.25, 0.01), dt=0.0025,
+                bodies=[(BOX, 0.5, 0.6, 0.7, 0.1, math.pi * 0.25, 0.0, 0.0, 0.0)], inflows=[], substeps=4)
+
+
 def config3_scene(seed=0x5EED0003):
     """Solver 7 (variable density): the reference box (7:1099), rho_air 0.1,
     rho_soot 1.0, diffusion 0.01, dt 0.005, and two inflows (0.45, 0.2) and
