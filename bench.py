@@ -42,8 +42,11 @@ def parse():
     ap.add_argument("--steps", type=int, default=4)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--size", type=int, default=None, help="grid side (default 2048; 1024 for --workload flip)")
-    ap.add_argument("--workload", choices=["solver7", "flip"], default="solver7",
-                    help="solver7: the headline 2048^2 variable-density step; flip: solver 8 at 1024^2")
+    ap.add_argument("--workload", choices=["solver7", "flip", "batch"], default="solver7",
+                    help="solver7: the headline 2048^2 variable-density step; flip: solver 8 at 1024^2; "
+                         "batch: 256 independent solver-4 instances of mixed sizes")
+    ap.add_argument("--threads", type=int, default=16, help="batch: host threads stepping the instances")
+    ap.add_argument("--instances", type=int, default=256, help="batch: number of instances")
     ap.add_argument("--form", choices=["fp64", "mixed"], default=None,
                     help="time only this form (default: FP64 as the value, the mixed form beside it)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -279,6 +282,91 @@ def run_flip(args):
     print(json.dumps(out))
 
 
+def cpu_baseline_batch(insts, sample):
+    """One update() of the first `sample` instances on one host core with the
+    reference's solver 4 (or the C restatement): cell-updates/s over them."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_lib as ol
+    kind = "reference" if ol.ref_available(4) else "port"
+    S = ol.RefSolver if kind == "reference" else ol.OracleSolver
+    cells, secs = 0, 0.0
+    for inst in insts[:sample]:
+        s = S(4, inst["w"], inst["h"], inst["params"], inst["bodies"])
+        for inf in inst["inflows"]:
+            s.add_inflow(*inf)
+        t0 = time.perf_counter()
+        s.update(inst["dt"])
+        secs += time.perf_counter() - t0
+        cells += inst["w"] * inst["h"]
+        if hasattr(s, "close"):
+            s.close()
+    return dict(value=cells / secs, unit="cell-updates/s", cores=1, kind=kind,
+                sample=f"first update() of the first {sample} instances ({cells} cells) on one host core: {secs:.1f} s")
+
+
+def run_batch(args):
+    """Config 4: independent solver-4 instances (sides drawn from 64..512,
+    one static box and one inflow each, ifl_scenes.config4_instances), each
+    a solver handle with its own HIP stream, stepped by `--threads` host
+    threads (the library releases the GIL in every call). Metric: total
+    cells x steps / s."""
+    import threading
+    import ifl_fluids as F
+    import ifl_scenes as S
+    insts = S.config4_instances(args.instances)
+    solvers = [F.FluidSolver(4, i["w"], i["h"], i["params"], [F.SolidBox(*b[1:]) for b in i["bodies"]])
+               for i in insts]
+    cells = sum(i["w"] * i["h"] for i in insts)
+    nt = max(1, args.threads)
+    errors = []
+
+    def run(idx, steps, iters):
+        try:
+            for _ in range(steps):
+                for k in idx:
+                    for inf in insts[k]["inflows"]:
+                        solvers[k].add_inflow(*inf)
+                    st = solvers[k].update(insts[k]["dt"])
+                    iters[k] += st.pressure.iterations
+        except Exception as e:  # reported, the run is invalid
+            errors.append(repr(e))
+
+    def step_all(steps):
+        iters = [0] * len(insts)
+        # largest instances first, dealt round robin so the threads finish together
+        order = sorted(range(len(insts)), key=lambda k: -insts[k]["w"] * insts[k]["h"])
+        th = [threading.Thread(target=run, args=(order[t::nt], steps, iters)) for t in range(nt)]
+        t0 = time.perf_counter()
+        for t in th:
+            t.start()
+        for t in th:
+            t.join()
+        return time.perf_counter() - t0, iters
+
+    step_all(args.warmup)
+    wall, iters = step_all(args.steps)
+    for sv in solvers:
+        sv.close()
+    if errors:
+        print(json.dumps({"error": errors[:3]}))
+        sys.exit(1)
+    out = {
+        "metric": "cell-updates/s, 256 independent solver-4 instances of mixed sizes, FP64 drop-in",
+        "value": cells * args.steps / wall, "unit": "cell-updates/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": 1e3 * wall / args.steps, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": f"solver4-batch-{len(insts)}-instances-64to512", "instances": len(insts),
+                   "cells": cells, "threads": nt, "streams": len(insts),
+                   "pressure_iterations_per_step": sum(iters) / args.steps},
+    }
+    if not args.no_cpu_baseline:
+        try:
+            out["cpu_baseline"] = cpu_baseline_batch(insts, 16)
+        except Exception as e:
+            out["cpu_baseline"] = {"error": str(e)}
+    print(json.dumps(out))
+
+
 def main():
     global cpu_iters_global
     args = parse()
@@ -292,6 +380,13 @@ def main():
         return
     if args.workload == "flip":
         run_flip(args)
+        return
+    if args.workload == "batch":
+        # one hardware queue per stepping thread (HIP's default is 4), set
+        # before the runtime starts; at most 16 sweeps of <= 8 strips are then
+        # in flight, well under one workgroup per CU
+        os.environ.setdefault("GPU_MAX_HW_QUEUES", str(min(16, max(4, args.threads))))
+        run_batch(args)
         return
     n = args.size or 2048
     # the headline value is the form asked for (FP64 by default); without

@@ -23,6 +23,8 @@
 // order (the sums are deterministic run to run; they differ from the
 // reference's left-to-right sum only by rounding, unless dot_order selects
 // the reference order).
+#include <atomic>
+
 #include "ifl_solver.hpp"
 
 namespace ifl {
@@ -923,11 +925,13 @@ static int launch_stream(Solver* s, StreamArgs a) {
     }
     a.nb = nb;
     size_t shmem = fixed + (size_t)nb * a.slot_bytes;
-    static bool attr_set = false;
-    if (!attr_set) {
+    // once per kernel; solvers stepped from several host threads may race
+    // here, which only repeats the same call
+    static std::atomic<bool> attr_set{false};
+    if (!attr_set.load(std::memory_order_acquire)) {
         IFL_HIP(hipFuncSetAttribute((const void*)stream_sweep_kernel<KIND, MODE, LV, NG, T>,
                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)shmem));
-        attr_set = true;
+        attr_set.store(true, std::memory_order_release);
     }
     // A strip waits for its upstream neighbour, so every strip of a launch
     // must be resident at once (one workgroup per CU): taller grids run as
