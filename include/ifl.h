@@ -165,6 +165,20 @@ int ifl_set_particle_count(ifl_solver* s, int count);
  * Call once after ifl_set_bodies; ifl_update calls it if it has not run. */
 int ifl_init_particles(ifl_solver* s);
 
+/* FluidSolver::toImage(rgba[, renderHeat]) 3:455 / 4:903 / 5:961 / 6:1021 /
+ * 7:1047 / 8:1426, rendered on the device: bytes = w*h*4, or w*h*8 with
+ * render_heat at levels 6-8 (the heat panel left of the density panel). */
+int ifl_to_image(ifl_solver* s, uint8_t* rgba, size_t bytes, int render_heat);
+
+/* The reference's frame file (lodepng_encode32_file, 7:1122): an RGBA, 8-bit
+ * PNG of w x h pixels (w doubled for a heat frame), stored deflate blocks. */
+int ifl_write_png(const char* path, const uint8_t* rgba, int w, int h);
+
+/* FluidSolver::maxTimestep (1:310-328), solver 1's CFL bound, at any level:
+ * 2h over the largest speed at the cell centres (each velocity component
+ * lerp'd from its faces), at most 1. Reads the current u, v. */
+int ifl_max_timestep(ifl_solver* s, double* dt);
+
 /* frand()'s seed (8:38-46): one per process, 0xBA5EBA11 at load, advanced by
  * every level-8 solver in creation and step order as in the reference. */
 uint32_t ifl_frand_state(void);

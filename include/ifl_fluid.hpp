@@ -213,54 +213,18 @@ public:
     double ambientT() { return _tAmb; }
 #endif
 
-    /* toImage 3:455-465 / 7:1047-1081: host-side rendering of the downloaded
-     * density (and temperature) fields */
+    /* toImage 3:455 / 4:903 / 5:961 / 6:1021 / 7:1047 / 8:1426, rendered on
+     * the device (ifl_to_image) into the caller's w*h*4 (w*h*8 with the heat
+     * panel) bytes */
 #if IFL_SOLVER_LEVEL <= 5
     void toImage(unsigned char* rgba) {
+        ifl_detail::check(ifl_to_image(_s, rgba, (size_t)_w * _h * 4, 0), _s);
+    }
 #else
     void toImage(unsigned char* rgba, bool renderHeat) {
-#endif
-        std::vector<double> d = field(IFL_FIELD_Q(IFL_Q_D, IFL_QF_SRC));
-        std::vector<double> vol;
-        if (IFL_SOLVER_LEVEL >= 5) vol = field(IFL_FIELD_Q(IFL_Q_D, IFL_QF_VOLUME));
-#if IFL_SOLVER_LEVEL >= 6
-        std::vector<double> t;
-        if (renderHeat) t = field(IFL_FIELD_Q(IFL_Q_T, IFL_QF_SRC));
-#endif
-        for (int y = 0; y < _h; y++) {
-            for (int x = 0; x < _w; x++) {
-                int i = x + y * _w;
-#if IFL_SOLVER_LEVEL <= 5
-                double v = IFL_SOLVER_LEVEL == 5 ? vol[i] : 1.0;
-                int shade = (int)((1.0 - d[i]) * v * 255.0);
-                shade = std::max(std::min(shade, 255), 0);
-                rgba[i * 4 + 0] = rgba[i * 4 + 1] = rgba[i * 4 + 2] = (unsigned char)shade;
-                rgba[i * 4 + 3] = 0xFF;
-#else
-                int idxl = 0, idxr;
-                if (renderHeat) {
-                    idxl = 4 * (x + y * _w * 2);
-                    idxr = 4 * (x + y * _w * 2 + _w);
-                } else
-                    idxr = 4 * i;
-                double volume = vol[i];
-                double shade = std::min(std::max((1.0 - d[i]) * volume, 0.0), 1.0);
-                rgba[idxr + 0] = rgba[idxr + 1] = rgba[idxr + 2] = (unsigned char)(int)(shade * 255.0);
-                rgba[idxr + 3] = 0xFF;
-                if (renderHeat) {
-                    double tt = std::min(std::max((t[i] - _tAmb) / 700.0, 0.0), 1.0);
-                    double r = 1.0 + volume * (std::min(tt * 4.0, 1.0) - 1.0);
-                    double g = 1.0 + volume * (std::min(tt * 2.0, 1.0) - 1.0);
-                    double b = 1.0 + volume * (std::max(std::min(tt * 4.0 - 3.0, 1.0), 0.0) - 1.0);
-                    rgba[idxl + 0] = (unsigned char)(int)(r * 255.0);
-                    rgba[idxl + 1] = (unsigned char)(int)(g * 255.0);
-                    rgba[idxl + 2] = (unsigned char)(int)(b * 255.0);
-                    rgba[idxl + 3] = 0xFF;
-                }
-#endif
-            }
-        }
+        ifl_detail::check(ifl_to_image(_s, rgba, (size_t)_w * _h * 4 * (renderHeat ? 2 : 1), renderHeat ? 1 : 0), _s);
     }
+#endif
 
     /* access for callers that read the state (the reference's fields are private) */
     ifl_solver* handle() { return _s; }

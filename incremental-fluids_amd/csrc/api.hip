@@ -1,10 +1,12 @@
 // C ABI of libifl.so (declared in include/ifl.h): solver lifetime, state
 // access, single stages and FluidSolver::update() for solvers 3..8.
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
 #include <mutex>
 #include <new>
+#include <vector>
 
 #include "ifl_solver.hpp"
 
@@ -182,7 +184,7 @@ void free_all(Solver* s) {
     F(s->stage_buf); F(s->front); F(s->front_cnt); F(s->bodies_d);
     F(s->posx); F(s->posy);
     for (int t = 0; t < 4; t++) F(s->prop[t]);
-    F(s->weight); F(s->counts);
+    F(s->weight); F(s->counts); F(s->image_d);
     flip_free(s);
     if (s->st_host) (void)hipHostFree(s->st_host);
     if (s->ev0) (void)hipEventDestroy(s->ev0);
@@ -408,6 +410,87 @@ int update_grid(Solver* s, double dt, ifl_step_stats* stats) {
     return IFL_OK;
 }
 
+// PNG (RGBA, 8 bits per channel, what the reference's lodepng_encode32_file
+// call writes, 7:1122) with the image data in stored deflate blocks: a valid zlib
+// stream (RFC 1950/1951) without compression, CRC-32 per chunk and Adler-32
+// over the filtered scanlines (filter byte 0 per row).
+struct CrcTable {
+    uint32_t t[256];
+    CrcTable() {
+        for (uint32_t i = 0; i < 256; i++) {
+            uint32_t k = i;
+            for (int b = 0; b < 8; b++) k = (k & 1) ? 0xEDB88320u ^ (k >> 1) : k >> 1;
+            t[i] = k;
+        }
+    }
+};
+
+uint32_t crc32_png(const uint8_t* p, size_t n, uint32_t c = 0xFFFFFFFFu) {
+    static const CrcTable table;  // initialised once, thread-safe
+    for (size_t i = 0; i < n; i++) c = table.t[(c ^ p[i]) & 0xFF] ^ (c >> 8);
+    return c;
+}
+
+void put32(std::vector<uint8_t>& o, uint32_t v) {
+    o.push_back((uint8_t)(v >> 24));
+    o.push_back((uint8_t)(v >> 16));
+    o.push_back((uint8_t)(v >> 8));
+    o.push_back((uint8_t)v);
+}
+
+void put_chunk(std::vector<uint8_t>& o, const char* type, const std::vector<uint8_t>& data) {
+    put32(o, (uint32_t)data.size());
+    const size_t start = o.size();
+    o.insert(o.end(), type, type + 4);
+    o.insert(o.end(), data.begin(), data.end());
+    put32(o, crc32_png(&o[start], o.size() - start) ^ 0xFFFFFFFFu);
+}
+
+bool encode_png(const uint8_t* rgba, int w, int h, std::vector<uint8_t>& out) {
+    const size_t row = (size_t)w * 4 + 1;
+    const size_t raw_n = row * (size_t)h;
+    if (raw_n / row != (size_t)h || raw_n > ((size_t)1 << 34)) return false;
+    std::vector<uint8_t> raw(raw_n);
+    for (int y = 0; y < h; y++) {
+        raw[y * row] = 0;  // filter: none
+        memcpy(&raw[y * row + 1], rgba + (size_t)y * w * 4, (size_t)w * 4);
+    }
+    std::vector<uint8_t> z;
+    z.push_back(0x78);  // deflate, 32 KB window
+    z.push_back(0x01);  // no preset dictionary, fastest; (0x78 << 8 | 0x01) % 31 == 0
+    for (size_t off = 0; off < raw_n || off == 0; off += 65535) {
+        const size_t len = std::min<size_t>(65535, raw_n - off);
+        const bool last = off + len >= raw_n;
+        z.push_back(last ? 1 : 0);  // BFINAL, BTYPE = 00 (stored)
+        z.push_back((uint8_t)len);
+        z.push_back((uint8_t)(len >> 8));
+        z.push_back((uint8_t)~len);
+        z.push_back((uint8_t)(~len >> 8));
+        z.insert(z.end(), raw.begin() + off, raw.begin() + off + len);
+        if (last) break;
+    }
+    uint32_t a = 1, b = 0;
+    for (size_t i = 0; i < raw_n; i++) {
+        a = (a + raw[i]) % 65521u;
+        b = (b + a) % 65521u;
+    }
+    put32(z, (b << 16) | a);
+    static const uint8_t sig[8] = {0x89, 'P', 'N', 'G', 0x0D, 0x0A, 0x1A, 0x0A};
+    out.assign(sig, sig + 8);
+    std::vector<uint8_t> ihdr;
+    put32(ihdr, (uint32_t)w);
+    put32(ihdr, (uint32_t)h);
+    ihdr.push_back(8);  // bit depth
+    ihdr.push_back(6);  // colour type: RGBA
+    ihdr.push_back(0);  // compression: deflate
+    ihdr.push_back(0);  // filter method 0
+    ihdr.push_back(0);  // no interlace
+    put_chunk(out, "IHDR", ihdr);
+    put_chunk(out, "IDAT", z);
+    put_chunk(out, "IEND", std::vector<uint8_t>());
+    return true;
+}
+
 }  // namespace
 
 extern "C" {
@@ -616,6 +699,50 @@ int ifl_init_particles(ifl_solver* h) {
     IFL_TRY(flip_init_particles(s));
     IFL_HIP(hipStreamSynchronize(s->stream));
     return IFL_OK;
+}
+
+int ifl_to_image(ifl_solver* h, uint8_t* rgba, size_t bytes, int render_heat) {
+    if (!h || !rgba) return IFL_ERR_ARG;
+    Solver* s = &h->s;
+    const int heat = render_heat && s->level >= 6;
+    const size_t need = (size_t)s->W * s->H * 4 * (heat ? 2 : 1);
+    if (bytes != need) return set_err(s, IFL_ERR_ARG, "ifl_to_image: buffer must hold w*h*4 bytes (w*h*8 with heat)");
+    IFL_HIP(hipSetDevice(s->device));
+    if (s->image_bytes < need) {
+        if (s->image_d) IFL_HIP(hipFree(s->image_d));
+        s->image_d = nullptr;
+        s->image_bytes = 0;
+        IFL_HIP(hipMalloc((void**)&s->image_d, need));
+        s->image_bytes = need;
+    }
+    IFL_TRY(k_to_image(s, s->image_d, heat));
+    IFL_HIP(hipMemcpyAsync(rgba, s->image_d, need, hipMemcpyDeviceToHost, s->stream));
+    IFL_HIP(hipStreamSynchronize(s->stream));
+    return IFL_OK;
+}
+
+int ifl_write_png(const char* path, const uint8_t* rgba, int w, int h) {
+    if (!path || !rgba || w <= 0 || h <= 0) return IFL_ERR_ARG;
+    std::vector<uint8_t> png;
+    if (!encode_png(rgba, w, h, png)) return IFL_ERR_ARG;
+    FILE* f = fopen(path, "wb");
+    if (!f) {
+        g_err = std::string("ifl_write_png: cannot open ") + path;
+        return IFL_ERR_ARG;
+    }
+    const bool ok = fwrite(png.data(), 1, png.size(), f) == png.size();
+    if (fclose(f) != 0 || !ok) {
+        g_err = std::string("ifl_write_png: write failed for ") + path;
+        return IFL_ERR_ARG;
+    }
+    return IFL_OK;
+}
+
+int ifl_max_timestep(ifl_solver* h, double* dt) {
+    if (!h || !dt) return IFL_ERR_ARG;
+    Solver* s = &h->s;
+    IFL_HIP(hipSetDevice(s->device));
+    return k_max_timestep(s, dt);
 }
 
 uint32_t ifl_frand_state(void) { return frand_state(); }

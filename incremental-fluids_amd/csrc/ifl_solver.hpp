@@ -110,6 +110,9 @@ struct Solver {
     std::vector<hipEvent_t> ev_free;  // event pool for per-launch timing
     std::vector<Timed> timed;         // recorded, not yet collected
 
+    uint8_t* image_d = nullptr;  // toImage output, allocated on first use
+    size_t image_bytes = 0;
+
     int border_solid = 0;
     size_t lds_bytes = 65536;  // dynamic LDS available to one workgroup
     int num_cus = 1;           // compute units: the strips of one sweep launch that can be resident at once
@@ -153,6 +156,8 @@ int k_set_boundary(Solver* s);
 int k_copy_t_to_r(Solver* s);
 int k_copy_p_to_t(Solver* s);
 int k_update_dflag(Solver* s);
+int k_max_timestep(Solver* s, double* dt_out);  // solver 1's CFL bound
+int k_to_image(Solver* s, uint8_t* dev_rgba, int heat);  // toImage into device memory
 // skewed <-> row-major conversion through s->stage_buf
 int k_skew_to_rows(Solver* s, const double* skew, double* rows);
 int k_rows_to_skew(Solver* s, const double* rows, double* skew);

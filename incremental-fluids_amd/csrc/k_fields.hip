@@ -181,6 +181,124 @@ int k_extrapolate(Solver* s, int qi) {
     return IFL_OK;
 }
 
+// ------------------------------------------------------------ maxTimestep
+// Solver 1's CFL bound (1:310-328): the largest speed at the cell centres.
+// A speed is sqrt of a sum of squares, so it is +0 or positive (or NaN, which
+// std::max never keeps); the maximum of such values does not depend on the
+// order, and their bit patterns order like the values, so one atomicMax on
+// the bits combines the waves exactly.
+__global__ __launch_bounds__(256) void max_speed_kernel(QtyView u, QtyView v, int W, int H,
+                                                         unsigned long long* __restrict__ out) {
+    long n = (long)W * H;
+    double m = 0.0;
+    for (long c = (long)blockIdx.x * blockDim.x + threadIdx.x; c < n; c += (long)gridDim.x * blockDim.x) {
+        int x = (int)(c % W), y = (int)(c / W);
+        double uu = qlerp(u.src, u.w, u.h, u.ox, u.oy, x + 0.5, y + 0.5);
+        double vv = qlerp(v.src, v.w, v.h, v.ox, v.oy, x + 0.5, y + 0.5);
+        double velocity = sqrt(uu * uu + vv * vv);
+        m = smax(m, velocity);
+    }
+    for (int off = 32; off > 0; off >>= 1) m = smax(m, __shfl_xor(m, off));
+    if ((threadIdx.x & 63) == 0) atomicMax(out, (unsigned long long)__double_as_longlong(m));
+}
+
+int k_max_timestep(Solver* s, double* dt_out) {
+    unsigned long long* d = (unsigned long long*)s->front_cnt;  // 8 bytes of scratch
+    IFL_HIP(hipMemsetAsync(d, 0, sizeof(unsigned long long), s->stream));
+    long n = (long)s->W * s->H;
+    int blocks = (int)std::min<long>(grid_blocks(n, 256), 4L * s->num_cus);
+    max_speed_kernel<<<blocks, 256, 0, s->stream>>>(s->q[IFL_Q_U], s->q[IFL_Q_V], s->W, s->H, d);
+    IFL_LAUNCH_CHECK();
+    unsigned long long bits = 0;
+    IFL_HIP(hipMemcpyAsync(&bits, d, sizeof bits, hipMemcpyDeviceToHost, s->stream));
+    IFL_HIP(hipStreamSynchronize(s->stream));
+    double max_velocity;
+    memcpy(&max_velocity, &bits, sizeof max_velocity);
+    double max_timestep = 2.0 * s->hx / max_velocity;  // 1:324
+    *dt_out = (1.0 < max_timestep) ? 1.0 : max_timestep;  // std::min(maxTimestep, 1.0), 1:327
+    return IFL_OK;
+}
+
+// ------------------------------------------------------------ toImage
+// 3:455-465 / 4:903-916 / 5:961-972 / 6:1021-1055 / 7:1047-1081 /
+// 8:1426-1467: one RGBA pixel (two with the heat panel) per cell. (int) of a
+// double follows x86 (NaN and out-of-range give INT_MIN); a byte store keeps
+// the low 8 bits of the int.
+IFL_DEV int x86_int(double v) {
+    return (v == v && v > -2147483649.0 && v < 2147483648.0) ? (int)v : (int)0x80000000u;
+}
+
+struct ImageArgs {
+    const double* d;
+    const double* volume;
+    const uint8_t* cell;
+    const double* t;
+    double t_amb;
+    int W, H, level, heat;
+    uint8_t* rgba;
+};
+
+__global__ void to_image_kernel(ImageArgs a) {
+    long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (long)a.W * a.H) return;
+    const int x = (int)(i % a.W), y = (int)(i / a.W), lv = a.level;
+    uint8_t* rgba = a.rgba;
+    if (lv <= 5) {
+        int shade = lv == 5 ? x86_int((1.0 - a.d[i]) * a.volume[i] * 255.0) : x86_int((1.0 - a.d[i]) * 255.0);
+        shade = imax(imin(shade, 255), 0);
+        if (lv == 4 && a.cell[i] == IFL_CELL_SOLID) shade = 0;  // 4:908-909
+        rgba[i * 4 + 0] = rgba[i * 4 + 1] = rgba[i * 4 + 2] = (uint8_t)shade;
+        rgba[i * 4 + 3] = 0xFF;
+        return;
+    }
+    long idxl = 0, idxr;
+    if (a.heat) {
+        idxl = 4 * ((long)x + (long)y * a.W * 2);
+        idxr = 4 * ((long)x + (long)y * a.W * 2 + a.W);
+    } else {
+        idxr = 4 * i;
+    }
+    const double volume = a.volume[i];
+    double shade = (1.0 - a.d[i]) * volume;
+    shade = smin(smax(shade, 0.0), 1.0);
+    const uint8_t g = (uint8_t)x86_int(shade * 255.0);
+    rgba[idxr + 0] = rgba[idxr + 1] = rgba[idxr + 2] = g;
+    rgba[idxr + 3] = 0xFF;
+    if (lv == 8 && a.cell[i] == IFL_CELL_EMPTY) {  // 8:1445-1448
+        rgba[idxr] = 0xFF;
+        rgba[idxr + 1] = rgba[idxr + 2] = 0;
+    }
+    if (a.heat) {
+        double t = lv == 8 ? fabs(a.t[i] - a.t_amb) / 70.0 : (a.t[i] - a.t_amb) / 700.0;  // 8:1451, 7:1067
+        t = smin(smax(t, 0.0), 1.0);
+        const double r = 1.0 + volume * (smin(t * 4.0, 1.0) - 1.0);
+        const double gg = 1.0 + volume * (smin(t * 2.0, 1.0) - 1.0);
+        const double b = 1.0 + volume * (smax(smin(t * 4.0 - 3.0, 1.0), 0.0) - 1.0);
+        rgba[idxl + 0] = (uint8_t)x86_int(r * 255.0);
+        rgba[idxl + 1] = (uint8_t)x86_int(gg * 255.0);
+        rgba[idxl + 2] = (uint8_t)x86_int(b * 255.0);
+        rgba[idxl + 3] = 0xFF;
+    }
+}
+
+int k_to_image(Solver* s, uint8_t* dev_rgba, int heat) {
+    ImageArgs a;
+    a.d = s->q[IFL_Q_D].src;
+    a.volume = s->q[IFL_Q_D].volume;
+    a.cell = s->q[IFL_Q_D].cell;
+    a.t = s->level >= 6 ? s->q[IFL_Q_T].src : nullptr;
+    a.t_amb = s->cfg.t_amb;
+    a.W = s->W;
+    a.H = s->H;
+    a.level = s->level;
+    a.heat = heat && s->level >= 6;
+    a.rgba = dev_rgba;
+    long n = (long)s->W * s->H;
+    to_image_kernel<<<grid_blocks(n, 256), 256, 0, s->stream>>>(a);
+    IFL_LAUNCH_CHECK();
+    return IFL_OK;
+}
+
 // ------------------------------------------------------------ advect
 struct AdvectArgs {
     QtyView q[4];

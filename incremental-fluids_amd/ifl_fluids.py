@@ -99,6 +99,9 @@ def _load():
     lib.ifl_get_particle_count.argtypes = [vp, ctypes.POINTER(ctypes.c_int)]
     lib.ifl_set_particle_count.argtypes = [vp, ctypes.c_int]
     lib.ifl_init_particles.argtypes = [vp]
+    lib.ifl_max_timestep.argtypes = [vp, ctypes.POINTER(ctypes.c_double)]
+    lib.ifl_to_image.argtypes = [vp, vp, ctypes.c_size_t, ctypes.c_int]
+    lib.ifl_write_png.argtypes = [ctypes.c_char_p, vp, ctypes.c_int, ctypes.c_int]
     lib.ifl_frand_state.restype = ctypes.c_uint32
     lib.ifl_frand_state.argtypes = []
     lib.ifl_frand_set_state.argtypes = [ctypes.c_uint32]
@@ -260,6 +263,19 @@ class FluidSolver:
     def synchronize(self):
         self._check(lib.ifl_synchronize(self._h))
 
+    def to_image(self, render_heat=False):
+        """toImage (7:1047): RGBA bytes, shape (h, w, 4) or (h, 2w, 4) with heat."""
+        heat = bool(render_heat) and self.level >= 6
+        a = np.empty((self.h, self.w * (2 if heat else 1), 4), dtype=np.uint8)
+        self._check(lib.ifl_to_image(self._h, a.ctypes.data, a.nbytes, 1 if heat else 0))
+        return a
+
+    def max_timestep(self):
+        """FluidSolver::maxTimestep (1:310-328): solver 1's CFL bound."""
+        dt = ctypes.c_double()
+        self._check(lib.ifl_max_timestep(self._h, ctypes.byref(dt)))
+        return dt.value
+
     def particle_count(self, value=None):
         """Level 8: the live particle count (_particleCount 8:701)."""
         if value is not None:
@@ -288,3 +304,12 @@ def reference_log_line(level, stats):
     if stats.status == SOLVE_INITIAL_SMALL and level >= 6:
         return "Initial guess sufficiently small\n"
     return ""
+
+
+def write_png(path, rgba):
+    """The reference's frame file (7:1122): an RGBA, 8-bit PNG."""
+    a = np.ascontiguousarray(rgba, dtype=np.uint8)
+    h, w = a.shape[0], a.shape[1]
+    rc = lib.ifl_write_png(os.fsencode(path), a.ctypes.data, w, h)
+    if rc != 0:
+        raise IflError(f"ifl_write_png failed rc={rc}: {lib.ifl_last_error(None).decode()}")

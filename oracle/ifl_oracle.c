@@ -1,6 +1,7 @@
 /*
  * ifl_oracle.c -- TEST INFRASTRUCTURE ONLY: a scalar C restatement of the
- * reference's FluidSolver::update() for solvers 3..8, used as the checker for
+ * reference's FluidSolver::update() for solvers 3..8 (and solver 1's
+ * maxTimestep, the CFL bound), used as the checker for
  * the HIP product and as the timed CPU baseline. Only tests/, the smoke entry
  * and bench.py's CPU-baseline leg load it; the product never does.
  *
@@ -12,7 +13,8 @@
  * Parity pins (tests/test_oracle_*.py):
  *   - bit-identical trajectories and stdout lines against the reference
  *     compiled from its own sources (oracle/_ref, built by oracle/Makefile),
- *     and against the committed fixtures under tests/golden/;
+ *     and against the committed fixtures under tests/golden/; maxTimestep
+ *     against solver 1 built the same way (oracle/_ref/libref1.so);
  *   - closed forms: occupancy of half-planes, SDFs, cubicPulse, frand's LCG,
  *     IC(0) factor identity at tau=0, dense-solve check of the MIC(0) sweeps
  *     and of the matrix-vector product on small grids.
@@ -1107,6 +1109,77 @@ static void qty_undiff(qty_t* q, double alpha) {
     for (long i = 0; i < (long)q->w * q->h; i++) q->src[i] += (1.0 - alpha) * q->dst[i];
 }
 
+/* maxTimestep 1:310-328 (solver 1): the largest speed at the cell centres,
+ * each component lerp'd from its faces, gives dt = 2h / speed, at most 1. */
+static double max_timestep(const orc_solver* s) {
+    double max_velocity = 0.0;
+    for (int y = 0; y < s->h; y++)
+        for (int x = 0; x < s->w; x++) {
+            double u = qty_lerp(&s->q[IFL_Q_U], x + 0.5, y + 0.5);
+            double v = qty_lerp(&s->q[IFL_Q_V], x + 0.5, y + 0.5);
+            double velocity = sqrt(u * u + v * v);
+            max_velocity = smax(max_velocity, velocity);
+        }
+    double max_timestep = 2.0 * s->hx / max_velocity;
+    return smin(max_timestep, 1.0);
+}
+
+/* (int) of a double as x86-64 converts it: NaN and out-of-range give INT_MIN */
+static int x86_int(double v) {
+    return (v == v && v > -2147483649.0 && v < 2147483648.0) ? (int)v : (int)0x80000000u;
+}
+
+/* toImage 3:455-465 / 4:903-916 / 5:961-972 / 6:1021-1055 / 7:1047-1081 /
+ * 8:1426-1467: RGBA bytes, w*h*4, or w*2*h*4 with the heat panel on the left
+ * (levels 6-8). Stores of an int into the byte array keep its low 8 bits. */
+static void to_image(const orc_solver* s, unsigned char* rgba, int render_heat) {
+    int w = s->w, h = s->h, lv = s->level;
+    const qty_t* d = &s->q[IFL_Q_D];
+    for (int y = 0; y < h; y++)
+        for (int x = 0; x < w; x++) {
+            int i = x + y * w;
+            if (lv <= 5) {
+                int shade = lv == 5 ? x86_int((1.0 - d->src[i]) * d->volume[i] * 255.0)
+                                    : x86_int((1.0 - d->src[i]) * 255.0);
+                shade = shade < 255 ? shade : 255;
+                shade = 0 < shade ? shade : 0;
+                if (lv == 4 && d->cell[i] == IFL_CELL_SOLID) shade = 0;
+                rgba[i * 4 + 0] = rgba[i * 4 + 1] = rgba[i * 4 + 2] = (unsigned char)shade;
+                rgba[i * 4 + 3] = 0xFF;
+                continue;
+            }
+            int idxl = 0, idxr;
+            if (render_heat) {
+                idxl = 4 * (x + y * w * 2);
+                idxr = 4 * (x + y * w * 2 + w);
+            } else {
+                idxr = 4 * i;
+            }
+            double volume = d->volume[i];
+            double shade = (1.0 - d->src[i]) * volume;
+            shade = smin(smax(shade, 0.0), 1.0);
+            unsigned char g = (unsigned char)x86_int(shade * 255.0);
+            rgba[idxr + 0] = rgba[idxr + 1] = rgba[idxr + 2] = g;
+            rgba[idxr + 3] = 0xFF;
+            if (lv == 8 && d->cell[i] == IFL_CELL_EMPTY) { /* 8:1445-1448 */
+                rgba[idxr] = 0xFF;
+                rgba[idxr + 1] = rgba[idxr + 2] = 0;
+            }
+            if (render_heat) {
+                double t = lv == 8 ? fabs(s->q[IFL_Q_T].src[i] - s->t_amb) / 70.0   /* 8:1451 */
+                                   : (s->q[IFL_Q_T].src[i] - s->t_amb) / 700.0;
+                t = smin(smax(t, 0.0), 1.0);
+                double r = 1.0 + volume * (smin(t * 4.0, 1.0) - 1.0);
+                double gg = 1.0 + volume * (smin(t * 2.0, 1.0) - 1.0);
+                double b = 1.0 + volume * (smax(smin(t * 4.0 - 3.0, 1.0), 0.0) - 1.0);
+                rgba[idxl + 0] = (unsigned char)x86_int(r * 255.0);
+                rgba[idxl + 1] = (unsigned char)x86_int(gg * 255.0);
+                rgba[idxl + 2] = (unsigned char)x86_int(b * 255.0);
+                rgba[idxl + 3] = 0xFF;
+            }
+        }
+}
+
 /* ---------------- public API (ctypes) ---------------- */
 
 orc_solver* orc_create(int level, int w, int h, const double* params, const orc_body_pod* bodies, int nbodies) {
@@ -1376,3 +1449,6 @@ double orc_body_distance_pod(const orc_body_pod* p, double x, double y) {
 }
 double orc_cubic_pulse(double x) { return cubic_pulse(x); }
 double orc_cerp1(double a, double b, double c, double d, double x) { return cerp1(a, b, c, d, x); }
+
+double orc_max_timestep(orc_solver* s) { return max_timestep(s); }
+void orc_to_image(orc_solver* s, unsigned char* rgba, int render_heat) { to_image(s, rgba, render_heat); }

@@ -171,6 +171,17 @@ void ref_add_inflow(void* p, double x, double y, double w, double h, double d, d
 
 void ref_update(void* p, double dt) { ((Handle*)p)->solver->update(dt); }
 
+/* toImage 3:455 / 4:903 / 5:961 / 6:1021 / 7:1047 / 8:1426; rgba holds
+ * w*h*4 bytes, or w*2*h*4 with render_heat (levels 6-8). */
+void ref_to_image(void* p, unsigned char* rgba, int render_heat) {
+#if REF_LEVEL <= 5
+    (void)render_heat;
+    ((Handle*)p)->solver->toImage(rgba);
+#else
+    ((Handle*)p)->solver->toImage(rgba, render_heat != 0);
+#endif
+}
+
 const char* ref_log(void) { return g_log.c_str(); }
 void ref_log_clear(void) { g_log.clear(); }
 

@@ -91,3 +91,70 @@ def test_stages_teacher_forced(level, w, h, bodies, warm):
                     orc.set_field(f, v)
     gpu.close()
     orc.close()
+
+
+def test_max_timestep_matches_oracle():
+    """maxTimestep (1:310-328) on the device against the oracle's restatement
+    (itself pinned to the reference's solver 1), on velocity fields from the
+    solver-7 scene and on random fields, bit for bit."""
+    import ctypes
+    import numpy as np
+    import ifl_fluids as F
+    lib = ol.oracle_lib()
+    lib.orc_max_timestep.restype = ctypes.c_double
+    lib.orc_max_timestep.argtypes = [ctypes.c_void_p]
+    sc = ol.REF_SCENES[7]
+    for w, h, steps in ((96, 80, 3), (130, 70, 2)):
+        orc = ol.OracleSolver(7, w, h, sc["params"], sc["bodies"], fresh=True)
+        for _ in range(steps):
+            orc.add_inflow(*sc["inflow"])
+            orc.update(sc["dt"])
+        gpu = make_product(7, w, h, sc["params"], sc["bodies"])
+        rng = np.random.default_rng(w)
+        fu, fv = ol.qfield(ol.Q_U, ol.QF_SRC), ol.qfield(ol.Q_V, ol.QF_SRC)
+        for case in ("scene", "random"):
+            if case == "random":
+                for f in (fu, fv):
+                    orc.set_field(f, rng.normal(0, 2.0, orc.field(f).shape))
+            for f in (fu, fv):
+                gpu.set_field(f, orc.field(f))
+            want = lib.orc_max_timestep(orc.h)
+            got = gpu.max_timestep()
+            print(f"{w}x{h} {case}: dt {got!r} (oracle {want!r})", flush=True)
+            assert np.float64(got).tobytes() == np.float64(want).tobytes()
+        gpu.close()
+        orc.close()
+
+
+@pytest.mark.parametrize("level", [3, 4, 5, 6, 7, 8])
+def test_to_image_matches_oracle(level, tmp_path):
+    """toImage rendered on the device against the oracle's restatement (itself
+    pinned to the reference's toImage), after a few steps, both panels for
+    solvers 6-8, byte for byte; and the frame file decodes to the same pixels."""
+    import ctypes
+    import numpy as np
+    import ifl_fluids as F
+    from test_png import decode
+    w, h = 64, 56
+    sc = ol.REF_SCENES[level]
+    if level == 8:
+        F.frand_set_state(0xBA5EBA11)
+    orc = ol.OracleSolver(level, w, h, sc["params"], sc["bodies"], fresh=True)
+    gpu = make_product(level, w, h, sc["params"], sc["bodies"], dot_order=1)
+    for _ in range(4):
+        for s in (orc, gpu):
+            if sc["inflow"] is not None:
+                s.add_inflow(*sc["inflow"])
+            s.update(sc["dt"])
+    lib = ol.oracle_lib()
+    lib.orc_to_image.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int]
+    for heat in ((0, 1) if level >= 6 else (0,)):
+        want = np.zeros((h, w * (2 if heat else 1), 4), dtype=np.uint8)
+        lib.orc_to_image(orc.h, want.ctypes.data, heat)
+        got = gpu.to_image(heat)
+        assert got.tobytes() == want.tobytes(), f"heat {heat}: {np.count_nonzero(got != want)} bytes differ"
+        p = tmp_path / f"frame{heat}.png"
+        F.write_png(str(p), got)
+        assert np.array_equal(decode(str(p)), want)
+    gpu.close()
+    orc.close()
