@@ -106,6 +106,9 @@ typedef struct ifl_step_stats {
     ifl_solve_stats pressure;  /* all levels */
     int32_t particle_count;    /* level 8, "Particle count: %d" 8:926 */
     int32_t border_solid;      /* 1 if a non-fluid cell touched the domain border */
+    int32_t nonfinite;         /* 1 if d, t, u or v holds a NaN or an infinity
+                                  after the step (the reference runs on silently) */
+    int32_t reserved;
 } ifl_step_stats;
 
 /* ---- lifetime: FluidSolver ctor 3:401 / 4:836 / 5:894 / 6:921 / 7:945 / 8:1289,

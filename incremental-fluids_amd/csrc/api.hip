@@ -184,7 +184,8 @@ void free_all(Solver* s) {
     F(s->stage_buf); F(s->front); F(s->front_cnt); F(s->bodies_d);
     F(s->posx); F(s->posy);
     for (int t = 0; t < 4; t++) F(s->prop[t]);
-    F(s->weight); F(s->counts); F(s->image_d);
+    F(s->weight); F(s->counts); F(s->image_d); F(s->step_flags);
+    if (s->step_flags_host) (void)hipHostFree(s->step_flags_host);
     flip_free(s);
     if (s->st_host) (void)hipHostFree(s->st_host);
     if (s->ev0) (void)hipEventDestroy(s->ev0);
@@ -278,6 +279,9 @@ int init_solver(Solver* s) {
     IFL_HIP(hipMalloc((void**)&s->front_cnt, 2 * sizeof(int)));
     IFL_HIP(hipMemsetAsync(s->front_cnt, 0, 2 * sizeof(int), s->stream));
     IFL_HIP(hipMalloc((void**)&s->bodies_d, kMaxBodies * sizeof(Body)));
+    IFL_HIP(hipMalloc((void**)&s->step_flags, 2 * sizeof(int)));
+    IFL_HIP(hipHostMalloc((void**)&s->step_flags_host, 2 * sizeof(int), hipHostMallocDefault));
+    s->step_flags_host[0] = s->step_flags_host[1] = 0;
     if (lv == 8) {
         // ParticleQuantities 8:865-878: room for 12 particles per cell; the
         // property arrays start at zero (8:896-897)
@@ -399,9 +403,13 @@ int update_grid(Solver* s, double dt, ifl_step_stats* stats) {
             flip_q(s, IFL_Q_V);
         }
     }
+    IFL_TRY(k_nonfinite(s, s->step_flags));
+    IFL_HIP(hipMemcpyAsync(s->step_flags_host, s->step_flags, sizeof(int), hipMemcpyDeviceToHost, s->stream));
     IFL_HIP(hipStreamSynchronize(s->stream));
     IFL_TRY(prof_collect(s, nullptr));
     if (stats) {
+        stats->nonfinite = s->step_flags_host[0];
+        stats->reserved = 0;
         stats->heat = heat;
         stats->pressure = press;
         stats->particle_count = s->particle_count;

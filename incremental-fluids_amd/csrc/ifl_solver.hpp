@@ -110,6 +110,8 @@ struct Solver {
     std::vector<hipEvent_t> ev_free;  // event pool for per-launch timing
     std::vector<Timed> timed;         // recorded, not yet collected
 
+    int* step_flags = nullptr;   // device: [0] non-finite field values after a step
+    int* step_flags_host = nullptr;  // pinned mirror
     uint8_t* image_d = nullptr;  // toImage output, allocated on first use
     size_t image_bytes = 0;
 
@@ -157,6 +159,7 @@ int k_copy_t_to_r(Solver* s);
 int k_copy_p_to_t(Solver* s);
 int k_update_dflag(Solver* s);
 int k_max_timestep(Solver* s, double* dt_out);  // solver 1's CFL bound
+int k_nonfinite(Solver* s, int* dev_flag);      // any NaN / infinity in d, t, u, v
 int k_to_image(Solver* s, uint8_t* dev_rgba, int heat);  // toImage into device memory
 // skewed <-> row-major conversion through s->stage_buf
 int k_skew_to_rows(Solver* s, const double* skew, double* rows);

@@ -181,6 +181,28 @@ int k_extrapolate(Solver* s, int qi) {
     return IFL_OK;
 }
 
+// ------------------------------------------------------------ non-finite check
+// Raises *flag when any of the n values is NaN or infinite.
+__global__ __launch_bounds__(256) void nonfinite_kernel(const double* __restrict__ a, long n, int* flag) {
+    bool bad = false;
+    for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
+        bad |= !isfinite(a[i]);
+    if (__any(bad) && (threadIdx.x & 63) == 0) atomicOr(flag, 1);
+}
+
+int k_nonfinite(Solver* s, int* dev_flag) {
+    IFL_HIP(hipMemsetAsync(dev_flag, 0, sizeof(int), s->stream));
+    for (int qi = 0; qi < 4; qi++) {
+        if (qi == IFL_Q_T && s->level < 6) continue;
+        const QtyView& q = s->q[qi];
+        long n = (long)q.w * q.h;
+        int blocks = (int)std::min<long>(grid_blocks(n, 256), 4L * s->num_cus);
+        nonfinite_kernel<<<blocks, 256, 0, s->stream>>>(q.src, n, dev_flag);
+        IFL_LAUNCH_CHECK();
+    }
+    return IFL_OK;
+}
+
 // ------------------------------------------------------------ maxTimestep
 // Solver 1's CFL bound (1:310-328): the largest speed at the cell centres.
 // A speed is sqrt of a sum of squares, so it is +0 or positive (or NaN, which

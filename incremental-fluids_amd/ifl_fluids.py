@@ -64,7 +64,7 @@ class SolveStats(ctypes.Structure):
 
 class StepStats(ctypes.Structure):
     _fields_ = [("heat", SolveStats), ("pressure", SolveStats), ("particle_count", ctypes.c_int32),
-                ("border_solid", ctypes.c_int32)]
+                ("border_solid", ctypes.c_int32), ("nonfinite", ctypes.c_int32), ("reserved", ctypes.c_int32)]
 
 
 class IflError(RuntimeError):

@@ -158,3 +158,21 @@ def test_to_image_matches_oracle(level, tmp_path):
         assert np.array_equal(decode(str(p)), want)
     gpu.close()
     orc.close()
+
+
+def test_step_reports_nonfinite_fields():
+    """ifl_step_stats.nonfinite: 0 on a normal step, 1 once a field holds a
+    NaN (the reference carries such values on without a word)."""
+    import numpy as np
+    sc = ol.REF_SCENES[7]
+    gpu = make_product(7, 64, 48, sc["params"], sc["bodies"])
+    gpu.add_inflow(*sc["inflow"])
+    st = gpu.update(sc["dt"])
+    assert st.nonfinite == 0
+    f = ol.qfield(ol.Q_D, ol.QF_SRC)
+    d = gpu.field(f)
+    d[50 + 10 * 64] = np.nan  # a fluid cell far from the box
+    gpu.set_field(f, d)
+    st = gpu.update(sc["dt"])
+    assert st.nonfinite == 1
+    gpu.close()
