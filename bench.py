@@ -385,7 +385,9 @@ def main():
         # one hardware queue per stepping thread (HIP's default is 4), set
         # before the runtime starts; at most 16 sweeps of <= 8 strips are then
         # in flight, well under one workgroup per CU
-        os.environ.setdefault("GPU_MAX_HW_QUEUES", str(min(16, max(4, args.threads))))
+        want = min(16, max(4, args.threads))
+        if int(os.environ.get("GPU_MAX_HW_QUEUES", "4")) < want:
+            os.environ["GPU_MAX_HW_QUEUES"] = str(want)
         run_batch(args)
         return
     n = args.size or 2048

@@ -766,13 +766,55 @@ IFL_DEV double block_sum_partials(const double* partial, int n) {
 }
 
 // dotProduct in the reference's left-to-right order (dot_order = SEQUENTIAL).
+// The reference's left-to-right sum (7:806-814, masked from level 6 on) in
+// row-major cell order, evaluated by one whole wave: the lanes load and
+// multiply 64 consecutive cells at a time, and the running sum takes their
+// products in lane order (every lane holds the same sum). Cells the
+// reference skips are skipped, not added as zero.
 IFL_DEV double seq_dot(const Skew& sk, const double* a, const double* b, const uint8_t* flag, int vm) {
-    double result = 0.0;
-    for (int y = 0; y < sk.H; y++)
-        for (int x = 0; x < sk.W; x++) {
-            long p = sk.pos(x, y);
-            if (!vm || flag[p]) result += a[p] * b[p];
+    constexpr int kU = 4;  // cells per lane per round: 256 consecutive cells
+    const int lane = threadIdx.x & 63;
+    const long n = (long)sk.W * sk.H;
+    double prod[2][kU];
+    bool inc[2][kU];
+    auto load = [&](long base, double (&pr)[kU], bool (&in)[kU]) {
+#pragma unroll
+        for (int u = 0; u < kU; u++) {
+            const long i = base + u * 64 + lane;
+            pr[u] = 0.0;
+            in[u] = false;
+            if (i < n) {
+                const int x = (int)(i % sk.W), y = (int)(i / sk.W);
+                const long p = sk.pos(x, y);
+                in[u] = !vm || flag[p];
+                if (in[u]) pr[u] = a[p] * b[p];
+            }
         }
+    };
+    double result = 0.0;
+    auto add = [&](const double (&pr)[kU], const bool (&in)[kU]) {
+#pragma unroll
+        for (int u = 0; u < kU; u++) {
+            const int lo = __double2loint(pr[u]), hi = __double2hiint(pr[u]);
+            uint64_t m = __ballot(in[u]);
+            while (m) {
+                const int k = __builtin_ctzll(m);
+                m &= m - 1;
+                result += __hiloint2double(__builtin_amdgcn_readlane(hi, k), __builtin_amdgcn_readlane(lo, k));
+            }
+        }
+    };
+    // two register sets: a round's loads go out before the previous round's
+    // additions
+    const long step = 64 * kU;
+    load(0, prod[0], inc[0]);
+    for (long base = 0; base < n; base += 2 * step) {
+        if (base + step < n) load(base + step, prod[1], inc[1]);
+        add(prod[0], inc[0]);
+        if (base + step >= n) break;
+        if (base + 2 * step < n) load(base + 2 * step, prod[0], inc[0]);
+        add(prod[1], inc[1]);
+    }
     return result;
 }
 
@@ -782,7 +824,10 @@ __global__ __launch_bounds__(256) void fin_alpha_kernel(VecArgs a, int n, int se
     if (st->done) return;
     double d;
     if (seq) {
-        if (threadIdx.x == 0) st->dot_tmp = seq_dot(a.sk, a.q, a.s, a.flag, a.vm);
+        if (threadIdx.x < 64) {
+            const double t = seq_dot(a.sk, a.q, a.s, a.flag, a.vm);
+            if (threadIdx.x == 0) st->dot_tmp = t;
+        }
         __syncthreads();
         d = st->dot_tmp;
     } else {
@@ -812,7 +857,10 @@ __global__ __launch_bounds__(256) void fin_sigma_kernel(VecArgs a, int n, int se
     if (!init && st->done) return;
     double d;
     if (seq) {
-        if (threadIdx.x == 0) st->dot_tmp = seq_dot(a.sk, a.z, a.r, a.flag, a.vm);
+        if (threadIdx.x < 64) {
+            const double t = seq_dot(a.sk, a.z, a.r, a.flag, a.vm);
+            if (threadIdx.x == 0) st->dot_tmp = t;
+        }
         __syncthreads();
         d = st->dot_tmp;
     } else {
@@ -879,7 +927,10 @@ __global__ void dot_partial_kernel(VecArgs a, const double* u, const double* v) 
 __global__ __launch_bounds__(256) void dot_finish_kernel(VecArgs a, const double* u, const double* v, int n, int seq, double* out) {
     double d;
     if (seq) {
-        if (threadIdx.x == 0) *out = seq_dot(a.sk, u, v, a.flag, a.vm);
+        if (threadIdx.x < 64) {
+            const double t = seq_dot(a.sk, u, v, a.flag, a.vm);
+            if (threadIdx.x == 0) *out = t;
+        }
         return;
     }
     d = block_sum_partials(a.partial, n);

@@ -1,0 +1,71 @@
+"""The workloads at their benchmark sizes.
+
+* Solver 8 (FLIP) at 1024x1024, the reference's scene (config 5): the first
+  update() from construction, ~4.1M particles, against the oracle bit for
+  bit (the product with the reference's dot order).
+* Solver 7 at 2048x2048, the benchmark scene (config 3), first update() with
+  the default tree-ordered dots: both solves meet the reference's stopping
+  test, the fields are finite, and the iteration counts are the reference's
+  own (stdout of the reference build, tests/golden/make_fullsize.py) to
+  within the rounding of the dot products.
+"""
+import json
+import os
+import re
+
+import numpy as np
+import pytest
+
+import oracle_lib as ol
+from gpu_util import compare_fields, make_product
+
+pytestmark = pytest.mark.gpu
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+
+
+def test_flip_1024_first_step_bit_exact():
+    import ifl_fluids as F
+    n = 1024
+    sc = ol.REF_SCENES[8]
+    F.frand_set_state(0xBA5EBA11)
+    orc = ol.OracleSolver(8, n, n, sc["params"], sc["bodies"], fresh=True)
+    gpu = make_product(8, n, n, sc["params"], sc["bodies"], dot_order=1)
+    orc.update(sc["dt"])
+    st = gpu.update(sc["dt"])
+    fields = ol.state_fields(8) + [ol.F_WEIGHT]
+    bad = compare_fields(orc, gpu, fields, where="1024x1024 FLIP step 0")
+    assert not bad, "\n".join(bad[:8])
+    assert st.particle_count == orc.particle_count() > 4_000_000
+    assert ol.oracle_lib().orc_frand_state() == F.frand_state()
+    print(f"1024x1024 FLIP: {st.particle_count} particles, heat {st.heat.iterations}, "
+          f"pressure {st.pressure.iterations} iterations, bit-exact", flush=True)
+    gpu.close()
+    orc.close()
+
+
+def test_solver7_2048_first_step():
+    import ifl_fluids as F
+    import ifl_scenes as S
+    with open(os.path.join(HERE, "golden", "solver7_2048_config3_step1.json")) as f:
+        ref = json.load(f)
+    iters = [int(x) for x in re.findall(r"Exiting solver after (\d+) iterations", ref["log"])]
+    assert len(iters) == 2  # heat, pressure
+    n = 2048
+    sc = S.config3_scene()
+    objs = [F.SolidBox(*b[1:]) for b in sc["bodies"]]
+    gpu = F.FluidSolver(7, n, n, sc["params"], objs)
+    for inf in sc["inflows"]:
+        gpu.add_inflow(*inf)
+    st = gpu.update(sc["dt"])
+    got = [st.heat.iterations, st.pressure.iterations]
+    print(f"2048x2048 solver 7 first step: iterations {got} (reference {iters}), "
+          f"max|r| {st.heat.max_error:.2e} / {st.pressure.max_error:.2e}", flush=True)
+    for s in (st.heat, st.pressure):
+        assert s.status == F.SOLVE_CONVERGED and s.max_error < 1e-5
+    assert st.nonfinite == 0
+    assert got[0] == iters[0]
+    # tree-ordered dots move the last bits of alpha and beta; over ~1500
+    # iterations of a nearly singular system the count may move slightly
+    assert abs(got[1] - iters[1]) <= max(2, iters[1] // 100)
+    gpu.close()
