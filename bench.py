@@ -47,6 +47,8 @@ def parse():
                          "batch: 256 independent solver-4 instances of mixed sizes")
     ap.add_argument("--threads", type=int, default=16, help="batch: host threads stepping the instances")
     ap.add_argument("--instances", type=int, default=256, help="batch: number of instances")
+    ap.add_argument("--batch-mode", choices=["fused", "streams"], default="fused",
+                    help="batch: one ifl_batch (PCG iterations fused over the instances) or one stream per instance")
     ap.add_argument("--form", choices=["fp64", "mixed"], default=None,
                     help="time only this form (default: FP64 as the value, the mixed form beside it)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -343,8 +345,25 @@ def run_batch(args):
             t.join()
         return time.perf_counter() - t0, iters
 
+    if args.batch_mode == "fused":
+        batch = F.Batch(solvers)
+
+        def step_all(steps):  # noqa: F811 - the fused form replaces the threaded one
+            iters = [0] * len(insts)
+            t0 = time.perf_counter()
+            for _ in range(steps):
+                for k in range(len(insts)):
+                    for inf in insts[k]["inflows"]:
+                        solvers[k].add_inflow(*inf)
+                st = batch.update(insts[0]["dt"])
+                for k, x in enumerate(st):
+                    iters[k] += x.pressure.iterations
+            return time.perf_counter() - t0, iters
+
     step_all(args.warmup)
     wall, iters = step_all(args.steps)
+    if args.batch_mode == "fused":
+        batch.close()
     for sv in solvers:
         sv.close()
     if errors:
@@ -356,7 +375,9 @@ def run_batch(args):
         "warmup": args.warmup, "ms_per_step": 1e3 * wall / args.steps, "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "f64", "data": "synthetic",
         "config": {"workload": f"solver4-batch-{len(insts)}-instances-64to512", "instances": len(insts),
-                   "cells": cells, "threads": nt, "streams": len(insts),
+                   "cells": cells, "mode": args.batch_mode,
+                   "threads": nt if args.batch_mode == "streams" else 1,
+                   "streams": len(insts) if args.batch_mode == "streams" else 1,
                    "pressure_iterations_per_step": sum(iters) / args.steps},
     }
     if not args.no_cpu_baseline:
@@ -381,13 +402,14 @@ def main():
     if args.workload == "flip":
         run_flip(args)
         return
-    if args.workload == "batch":
+    if args.workload == "batch" and args.batch_mode == "streams":
         # one hardware queue per stepping thread (HIP's default is 4), set
         # before the runtime starts; at most 16 sweeps of <= 8 strips are then
         # in flight, well under one workgroup per CU
         want = min(16, max(4, args.threads))
         if int(os.environ.get("GPU_MAX_HW_QUEUES", "4")) < want:
             os.environ["GPU_MAX_HW_QUEUES"] = str(want)
+    if args.workload == "batch":
         run_batch(args)
         return
     n = args.size or 2048

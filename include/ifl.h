@@ -168,6 +168,18 @@ int ifl_set_particle_count(ifl_solver* s, int count);
  * Call once after ifl_set_bodies; ifl_update calls it if it has not run. */
 int ifl_init_particles(ifl_solver* s);
 
+/* ---- batches: independent solvers stepped together ----
+ * The solvers (one level, 3..7, one device, FP64 form, tree-ordered dots)
+ * keep their own state; ifl_batch_update runs FluidSolver::update(dt) on all
+ * of them (the reference steps each in turn, 7:1110-1126), with each PCG
+ * iteration of their project() calls one launch per phase for the whole
+ * batch. stats receives n entries. No counterpart in the reference: a
+ * caller with many small simulations would otherwise step them one by one. */
+typedef struct ifl_batch ifl_batch;
+int ifl_batch_create(ifl_solver* const* solvers, int n, ifl_batch** out);
+int ifl_batch_update(ifl_batch* batch, double dt, ifl_step_stats* stats);
+int ifl_batch_destroy(ifl_batch* batch);
+
 /* FluidSolver::toImage(rgba[, renderHeat]) 3:455 / 4:903 / 5:961 / 6:1021 /
  * 7:1047 / 8:1426, rendered on the device: bytes = w*h*4, or w*h*8 with
  * render_heat at levels 6-8 (the heat panel left of the density panel). */

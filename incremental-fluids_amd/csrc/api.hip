@@ -16,6 +16,14 @@ struct ifl_solver {
     Solver s;
 };
 
+struct ifl_batch {
+    std::vector<ifl_solver*> handles;
+    std::vector<Solver*> solvers;
+    hipStream_t stream = nullptr;
+    int device = 0;
+    ifl::BatchDev* dev = nullptr;
+};
+
 namespace ifl {
 
 // frand() 8:38-46 keeps one static seed for the whole process; every level-8
@@ -333,22 +341,22 @@ int fill_solid(Solver* s, int qi) {
 }
 
 // FluidSolver::update for levels 3..8 (3:433-447, 4:869-895, 5:927-953,
-// 6:966-1008, 7:995-1034, 8:1350-1413)
-int update_grid(Solver* s, double dt, ifl_step_stats* stats) {
-    int lv = s->level;
-    ifl_solve_stats heat = {IFL_SOLVE_NOT_RUN, 0, 0.0}, press = heat;
-    s->border_solid = 0;
-    if (lv == 3) {
-        IFL_TRY(k_build_rhs(s));
-        IFL_TRY(k_build_pressure(s, dt));
-        IFL_TRY(k_build_precon(s));
-        IFL_TRY(solve(s, &press));
-        IFL_TRY(k_apply_pressure(s, dt));
-        IFL_TRY(k_advect(s, (1 << IFL_Q_D) | (1 << IFL_Q_U) | (1 << IFL_Q_V), dt));
-        flip_q(s, IFL_Q_D);
-        flip_q(s, IFL_Q_U);
-        flip_q(s, IFL_Q_V);
-    } else {
+// 6:966-1008, 7:995-1034, 8:1350-1413), split at its linear solves so that a
+// batch of solvers can run each solve together: update_part runs from the
+// solve `after` (SOLVE_NONE at the start of the step) up to the next one and
+// names it in *next (SOLVE_NONE once the step is complete).
+int update_part(Solver* s, int after, double dt, int* next) {
+    const int lv = s->level;
+    *next = SOLVE_NONE;
+    if (after == SOLVE_NONE) {
+        s->border_solid = 0;
+        if (lv == 3) {
+            IFL_TRY(k_build_rhs(s));
+            IFL_TRY(k_build_pressure(s, dt));
+            IFL_TRY(k_build_precon(s));
+            *next = SOLVE_PRESSURE;
+            return IFL_OK;
+        }
         if (lv == 8 && !s->particles_ready) IFL_TRY(flip_init_particles(s));
         IFL_TRY(fill_solid(s, IFL_Q_D));
         if (lv >= 6) IFL_TRY(fill_solid(s, IFL_Q_T));
@@ -369,40 +377,58 @@ int update_grid(Solver* s, double dt, ifl_step_stats* stats) {
             IFL_TRY(k_copy_t_to_r(s));
             IFL_TRY(k_build_heat(s, dt));
             IFL_TRY(k_build_precon(s));
-            IFL_TRY(solve(s, &heat));
-            IFL_TRY(k_copy_p_to_t(s));
-            IFL_TRY(extrapolate(s, IFL_Q_T));
-            IFL_TRY(k_add_buoyancy(s, dt));
+            *next = SOLVE_HEAT;
+            return IFL_OK;
         }
+    }
+    if (after == SOLVE_HEAT) {
+        IFL_TRY(k_copy_p_to_t(s));
+        IFL_TRY(extrapolate(s, IFL_Q_T));
+        IFL_TRY(k_add_buoyancy(s, dt));
+    }
+    if (after != SOLVE_PRESSURE) {
         IFL_TRY(k_set_boundary(s));
         IFL_TRY(k_build_rhs(s));
         if (lv >= 7) IFL_TRY(k_compute_densities(s));
         IFL_TRY(k_build_pressure(s, dt));
         IFL_TRY(k_build_precon(s));
-        IFL_TRY(solve(s, &press));
-        IFL_TRY(k_apply_pressure(s, dt));
-        IFL_TRY(extrapolate(s, IFL_Q_D));
-        IFL_TRY(extrapolate(s, IFL_Q_U));
-        IFL_TRY(extrapolate(s, IFL_Q_V));
-        if (lv < 8) IFL_TRY(read_border_flag(s));
-        IFL_TRY(k_set_boundary(s));
-        if (lv == 8) {
-            // 8:1393-1412: FLIP update of the particles, then their advection
-            const double alpha = s->cfg.flip_alpha;
-            for (int qi = 0; qi < 4; qi++) IFL_TRY(k_diff(s, qi, alpha, 0));
-            IFL_TRY(k_grid_to_particles(s, alpha));
-            for (int qi = 0; qi < 4; qi++) IFL_TRY(k_diff(s, qi, alpha, 1));
-            IFL_TRY(k_particle_advect(s, dt));
-            IFL_TRY(flip_check_overflow(s));
-        } else {
-            int qm = (1 << IFL_Q_D) | (1 << IFL_Q_U) | (1 << IFL_Q_V) | (lv >= 6 ? (1 << IFL_Q_T) : 0);
-            IFL_TRY(k_advect(s, qm, dt));
-            flip_q(s, IFL_Q_D);
-            if (lv >= 6) flip_q(s, IFL_Q_T);
-            flip_q(s, IFL_Q_U);
-            flip_q(s, IFL_Q_V);
-        }
+        *next = SOLVE_PRESSURE;
+        return IFL_OK;
     }
+    IFL_TRY(k_apply_pressure(s, dt));
+    if (lv == 3) {
+        IFL_TRY(k_advect(s, (1 << IFL_Q_D) | (1 << IFL_Q_U) | (1 << IFL_Q_V), dt));
+        flip_q(s, IFL_Q_D);
+        flip_q(s, IFL_Q_U);
+        flip_q(s, IFL_Q_V);
+        return IFL_OK;
+    }
+    IFL_TRY(extrapolate(s, IFL_Q_D));
+    IFL_TRY(extrapolate(s, IFL_Q_U));
+    IFL_TRY(extrapolate(s, IFL_Q_V));
+    if (lv < 8) IFL_TRY(read_border_flag(s));
+    IFL_TRY(k_set_boundary(s));
+    if (lv == 8) {
+        // 8:1393-1412: FLIP update of the particles, then their advection
+        const double alpha = s->cfg.flip_alpha;
+        for (int qi = 0; qi < 4; qi++) IFL_TRY(k_diff(s, qi, alpha, 0));
+        IFL_TRY(k_grid_to_particles(s, alpha));
+        for (int qi = 0; qi < 4; qi++) IFL_TRY(k_diff(s, qi, alpha, 1));
+        IFL_TRY(k_particle_advect(s, dt));
+        IFL_TRY(flip_check_overflow(s));
+    } else {
+        int qm = (1 << IFL_Q_D) | (1 << IFL_Q_U) | (1 << IFL_Q_V) | (lv >= 6 ? (1 << IFL_Q_T) : 0);
+        IFL_TRY(k_advect(s, qm, dt));
+        flip_q(s, IFL_Q_D);
+        if (lv >= 6) flip_q(s, IFL_Q_T);
+        flip_q(s, IFL_Q_U);
+        flip_q(s, IFL_Q_V);
+    }
+    return IFL_OK;
+}
+
+// the end of a step: the non-finite check, timing collection, statistics
+int finish_step(Solver* s, const ifl_solve_stats& heat, const ifl_solve_stats& press, ifl_step_stats* stats) {
     IFL_TRY(k_nonfinite(s, s->step_flags));
     IFL_HIP(hipMemcpyAsync(s->step_flags_host, s->step_flags, sizeof(int), hipMemcpyDeviceToHost, s->stream));
     IFL_HIP(hipStreamSynchronize(s->stream));
@@ -416,6 +442,18 @@ int update_grid(Solver* s, double dt, ifl_step_stats* stats) {
         stats->border_solid = s->border_solid;
     }
     return IFL_OK;
+}
+
+int update_grid(Solver* s, double dt, ifl_step_stats* stats) {
+    ifl_solve_stats heat = {IFL_SOLVE_NOT_RUN, 0, 0.0}, press = heat;
+    int after = SOLVE_NONE, next = SOLVE_NONE;
+    while (true) {
+        IFL_TRY(update_part(s, after, dt, &next));
+        if (next == SOLVE_NONE) break;
+        IFL_TRY(solve(s, next == SOLVE_HEAT ? &heat : &press));
+        after = next;
+    }
+    return finish_step(s, heat, press, stats);
 }
 
 // PNG (RGBA, 8 bits per channel, what the reference's lodepng_encode32_file
@@ -743,6 +781,88 @@ int ifl_write_png(const char* path, const uint8_t* rgba, int w, int h) {
         g_err = std::string("ifl_write_png: write failed for ") + path;
         return IFL_ERR_ARG;
     }
+    return IFL_OK;
+}
+
+int ifl_batch_create(ifl_solver* const* handles, int n, ifl_batch** out) {
+    if (!handles || !out || n < 1) return IFL_ERR_ARG;
+    *out = nullptr;
+    const Solver* s0 = handles[0] ? &handles[0]->s : nullptr;
+    if (!s0) return IFL_ERR_ARG;
+    for (int i = 0; i < n; i++) {
+        const Solver* s = handles[i] ? &handles[i]->s : nullptr;
+        if (!s) return IFL_ERR_ARG;
+        if (s->level != s0->level || s->level == 8 || s->device != s0->device ||
+            s->cfg.precision != IFL_PREC_FP64 || s->cfg.dot_order != IFL_DOT_TREE) {
+            g_err = "ifl_batch_create: solvers of one level (3-7) on one device, FP64 form, tree-ordered dots";
+            return IFL_ERR_UNSUPPORTED;
+        }
+        for (int k = 0; k < i; k++)
+            if (handles[k] == handles[i]) {
+                g_err = "ifl_batch_create: a solver appears twice";
+                return IFL_ERR_ARG;
+            }
+    }
+    ifl_batch* b = new (std::nothrow) ifl_batch();
+    if (!b) return IFL_ERR_ALLOC;
+    b->handles.assign(handles, handles + n);
+    for (int i = 0; i < n; i++) b->solvers.push_back(&handles[i]->s);
+    b->device = s0->device;
+    if (hipSetDevice(b->device) != hipSuccess || hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess) {
+        g_err = "ifl_batch_create: stream creation failed";
+        delete b;
+        return IFL_ERR_HIP;
+    }
+    *out = b;
+    return IFL_OK;
+}
+
+int ifl_batch_update(ifl_batch* b, double dt, ifl_step_stats* stats) {
+    if (!b) return IFL_ERR_ARG;
+    const int n = (int)b->solvers.size();
+    if (hipSetDevice(b->device) != hipSuccess) return IFL_ERR_HIP;
+    // every stage of every solver goes to the batch's stream, in order
+    std::vector<hipStream_t> own(n);
+    for (int i = 0; i < n; i++) {
+        own[i] = b->solvers[i]->stream;
+        if (hipStreamSynchronize(own[i]) != hipSuccess) return IFL_ERR_HIP;
+    }
+    for (int i = 0; i < n; i++) b->solvers[i]->stream = b->stream;
+    const ifl_solve_stats none = {IFL_SOLVE_NOT_RUN, 0, 0.0};
+    std::vector<ifl_solve_stats> heat(n, none), press(n, none);
+    int rc = IFL_OK, after = SOLVE_NONE;
+    while (rc == IFL_OK) {
+        int next = SOLVE_NONE;
+        for (int i = 0; i < n && rc == IFL_OK; i++) {
+            int ni = SOLVE_NONE;
+            rc = update_part(b->solvers[i], after, dt, &ni);
+            if (rc != IFL_OK) g_err = b->solvers[i]->err;
+            next = ni;  // every solver of one level takes the same sequence of solves
+        }
+        if (rc != IFL_OK || next == SOLVE_NONE) break;
+        rc = k_project_batch(b->solvers.data(), n, b->stream, &b->dev,
+                             next == SOLVE_HEAT ? heat.data() : press.data());
+        if (rc != IFL_OK) g_err = b->solvers[0]->err;
+        after = next;
+    }
+    for (int i = 0; i < n && rc == IFL_OK; i++) {
+        rc = finish_step(b->solvers[i], heat[i], press[i], stats ? &stats[i] : nullptr);
+        if (rc != IFL_OK) g_err = b->solvers[i]->err;
+    }
+    (void)hipStreamSynchronize(b->stream);
+    for (int i = 0; i < n; i++) b->solvers[i]->stream = own[i];
+    return rc;
+}
+
+int ifl_batch_destroy(ifl_batch* b) {
+    if (!b) return IFL_OK;
+    (void)hipSetDevice(b->device);
+    if (b->stream) {
+        (void)hipStreamSynchronize(b->stream);
+        (void)hipStreamDestroy(b->stream);
+    }
+    batch_dev_free(b->dev);
+    delete b;
     return IFL_OK;
 }
 

@@ -165,6 +165,9 @@ int k_to_image(Solver* s, uint8_t* dev_rgba, int heat);  // toImage into device 
 int k_skew_to_rows(Solver* s, const double* skew, double* rows);
 int k_rows_to_skew(Solver* s, const double* rows, double* skew);
 
+// the linear solves of one update() (FluidSolver::project calls)
+enum { SOLVE_NONE = 0, SOLVE_HEAT = 1, SOLVE_PRESSURE = 2 };
+
 // ---- launchers (k_pcg.hip) ----
 int k_build_precon(Solver* s);
 int k_apply_precon(Solver* s);       // z = M^-1 r (fluid cells), the stage form
@@ -172,6 +175,11 @@ int k_matvec(Solver* s);             // z = A s, the stage form
 int k_dot(Solver* s, int which, double* out);  // 0: z.r  1: z.s
 int k_inf_norm(Solver* s, double* out);
 int k_project(Solver* s, int limit, ifl_solve_stats* stats);
+// project() of several solvers of one level at once, all on `stream`: one
+// launch per phase of each PCG iteration for all of them (stats: n entries)
+struct BatchDev;
+int k_project_batch(Solver* const* ss, int n, hipStream_t stream, BatchDev** dev, ifl_solve_stats* stats);
+void batch_dev_free(BatchDev* d);
 // per-launch timing (when profiling): events on the solver's stream around
 // a launch; prof_collect() adds the elapsed times of launches whose index is
 // below the limit given for their kernel, after the stream has completed.

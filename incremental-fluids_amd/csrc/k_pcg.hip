@@ -23,7 +23,9 @@
 // order (the sums are deterministic run to run; they differ from the
 // reference's left-to-right sum only by rounding, unless dot_order selects
 // the reference order).
+#include <algorithm>
 #include <atomic>
+#include <vector>
 
 #include "ifl_solver.hpp"
 
@@ -627,10 +629,8 @@ IFL_DEV void writer_wave(const StreamArgs& a, char* ring, RingHdr* hdr, int j, i
 }
 
 template <int KIND, int MODE, int LV, int NG, typename T>
-__global__ __launch_bounds__(64 * kWaves) void stream_sweep_kernel(StreamArgs a) {
+IFL_DEV void sweep_body(const StreamArgs& a, int j) {
     extern __shared__ __align__(16) char smem[];
-    const int j = strip_of_block(blockIdx.x, a.strip0, a.nlaunch);
-    if (j < 0) return;
     PcgState* st = a.st;
     if (KIND != SW_PRECON && ((MODE == 1 && st->done) || st->timeout)) return;
     RingHdr* hdr = (RingHdr*)smem;
@@ -656,6 +656,13 @@ __global__ __launch_bounds__(64 * kWaves) void stream_sweep_kernel(StreamArgs a)
         poller_wave<KIND>(a, ein, ering, hdr, lane);
     else if (wave >= kWaveLoader0 && wave < kWaveLoader0 + kLoaders)
         loader_wave<KIND, MODE, LV, NG, T>(a, ring, hdr, wave - kWaveLoader0, lane, sbase, nblk, j);
+}
+
+template <int KIND, int MODE, int LV, int NG, typename T>
+__global__ __launch_bounds__(64 * kWaves) void stream_sweep_kernel(StreamArgs a) {
+    const int j = strip_of_block(blockIdx.x, a.strip0, a.nlaunch);
+    if (j < 0) return;
+    sweep_body<KIND, MODE, LV, NG, T>(a, j);
 }
 
 // ---------------------------------------------------------------- vector phases
@@ -685,11 +692,11 @@ IFL_DEV bool skew_cell(const Skew& sk, long i, int& x, int& y) {
 // The reference's _z at a non-FLUID cell is still A s from the matrix-vector
 // product (applyPreconditioner skips the cell, 4:766-796); this product keeps
 // A s in q, so levels 4-5 (unmasked scaledAdd) read it from there.
-__global__ void vec_update_kernel(VecArgs a, int mode, int check_done) {
+IFL_DEV void vec_update_body(const VecArgs& a, long blk, int mode, int check_done) {
     PcgState* st = a.st;
     if (check_done && st->done) return;
     long n = (long)a.sk.nstrips * a.sk.strip_sz;
-    long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    long i = blk * (long)blockDim.x + threadIdx.x;
     if (i >= n) return;
     int x, y;
     if (!skew_cell(a.sk, i, x, y)) return;
@@ -699,6 +706,7 @@ __global__ void vec_update_kernel(VecArgs a, int mode, int check_done) {
     if (mode != 1) a.p[i] = a.p[i] + s * st->alpha;
     if (mode != 2) a.s[i] = (a.pm && !fl ? a.q[i] : a.z[i]) + s * st->beta;
 }
+__global__ void vec_update_kernel(VecArgs a, int mode, int check_done) { vec_update_body(a, blockIdx.x, mode, check_done); }
 
 // End of a solve that ran out of its budget: the reference's _z holds A s
 // at every non-FLUID cell (levels 4+), the preconditioned residual elsewhere.
@@ -715,13 +723,13 @@ __global__ void z_nonfluid_kernel(VecArgs a) {
 // neighbour terms in the reference's order, skipped at the domain border)
 // fused with the block partial of dotProduct(q, s).
 template <int OUT_Q>
-__global__ __launch_bounds__(256) void matvec_kernel(VecArgs a, int check_done) {
+IFL_DEV void matvec_body(const VecArgs& a, long blk, int check_done) {
     __shared__ double red[4];
     PcgState* st = a.st;
     if (check_done && st->done) return;
     const Skew& sk = a.sk;
     long n = (long)sk.nstrips * sk.strip_sz;
-    long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    long i = blk * (long)blockDim.x + threadIdx.x;
     double prod = 0.0;
     int x, y;
     if (i < n && skew_cell(sk, i, x, y)) {
@@ -748,7 +756,11 @@ __global__ __launch_bounds__(256) void matvec_kernel(VecArgs a, int check_done) 
     prod = wave_sum(prod);
     if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = prod;
     __syncthreads();
-    if (threadIdx.x == 0) a.partial[blockIdx.x] = (red[0] + red[1]) + (red[2] + red[3]);
+    if (threadIdx.x == 0) a.partial[blk] = (red[0] + red[1]) + (red[2] + red[3]);
+}
+template <int OUT_Q>
+__global__ __launch_bounds__(256) void matvec_kernel(VecArgs a, int check_done) {
+    matvec_body<OUT_Q>(a, blockIdx.x, check_done);
 }
 
 // Fixed-order sum of `n` partials by one workgroup of 256 threads.
@@ -819,7 +831,7 @@ IFL_DEV double seq_dot(const Skew& sk, const double* a, const double* b, const u
 }
 
 // finish A2: alpha = sigma / dot(q, s)  (7:868)
-__global__ __launch_bounds__(256) void fin_alpha_kernel(VecArgs a, int n, int seq) {
+IFL_DEV void fin_alpha_body(const VecArgs& a, int n, int seq) {
     PcgState* st = a.st;
     if (st->done) return;
     double d;
@@ -838,9 +850,10 @@ __global__ __launch_bounds__(256) void fin_alpha_kernel(VecArgs a, int n, int se
         st->max_err = 0.0;
     }
 }
+__global__ __launch_bounds__(256) void fin_alpha_kernel(VecArgs a, int n, int seq) { fin_alpha_body(a, n, seq); }
 
 // finish B: the convergence test of 7:872-876
-__global__ void fin_check_kernel(PcgState* st, double tol) {
+IFL_DEV void fin_check_body(PcgState* st, double tol) {
     if (st->done) return;
     if (st->max_err < tol) {
         st->done = 1;
@@ -848,11 +861,12 @@ __global__ void fin_check_kernel(PcgState* st, double tol) {
         st->printed_iter = st->iter;
     }
 }
+__global__ void fin_check_kernel(PcgState* st, double tol) { fin_check_body(st, tol); }
 
 // finish C: sigmaNew = dot(z, r), beta = sigmaNew/sigma (7:880-882).
 // init=1 is the entry of project(): sigma = dot(z, r) and the early-exit
 // test of 7:858-862.
-__global__ __launch_bounds__(256) void fin_sigma_kernel(VecArgs a, int n, int seq, int init, double tol, int limit) {
+IFL_DEV void fin_sigma_body(const VecArgs& a, int n, int seq, int init, double tol, int limit) {
     PcgState* st = a.st;
     if (!init && st->done) return;
     double d;
@@ -889,6 +903,9 @@ __global__ __launch_bounds__(256) void fin_sigma_kernel(VecArgs a, int n, int se
         st->status = IFL_SOLVE_BUDGET;
         st->printed_iter = limit;
     }
+}
+__global__ __launch_bounds__(256) void fin_sigma_kernel(VecArgs a, int n, int seq, int init, double tol, int limit) {
+    fin_sigma_body(a, n, seq, init, tol, limit);
 }
 
 __global__ void state_reset_kernel(PcgState* st) {
@@ -967,7 +984,7 @@ static StreamArgs stream_args(Solver* s, int kind) {
 }
 
 template <int KIND, int MODE, int LV, int NG, typename T>
-static int launch_stream(Solver* s, StreamArgs a) {
+static int ring_setup(Solver* s, StreamArgs& a, size_t* shmem) {
     a.slot_bytes = SweepShape<KIND, MODE, NG, T>::SLOT;
     size_t fixed = sizeof(RingHdr) + kERing * sizeof(double);
     int nb = (int)((s->lds_bytes - fixed) / a.slot_bytes);
@@ -977,7 +994,14 @@ static int launch_stream(Solver* s, StreamArgs a) {
         return IFL_ERR_UNSUPPORTED;
     }
     a.nb = nb;
-    size_t shmem = fixed + (size_t)nb * a.slot_bytes;
+    *shmem = fixed + (size_t)nb * a.slot_bytes;
+    return IFL_OK;
+}
+
+template <int KIND, int MODE, int LV, int NG, typename T>
+static int launch_stream(Solver* s, StreamArgs a) {
+    size_t shmem = 0;
+    IFL_TRY((ring_setup<KIND, MODE, LV, NG, T>(s, a, &shmem)));
     // once per kernel; solvers stepped from several host threads may race
     // here, which only repeats the same call
     static std::atomic<bool> attr_set{false};
@@ -1004,7 +1028,7 @@ static bool mixed(const Solver* s) { return s->cfg.precision == IFL_PREC_MIXED; 
 // forward sweep: the drop-in reads the double factor, the mixed form its
 // single-precision copy (and writes y in single precision)
 template <int MODE, typename T>
-static int launch_fwd_t(Solver* s, long long* trace) {
+static StreamArgs fwd_args(Solver* s) {
     StreamArgs a = stream_args(s, SW_FWD);
     const bool sp = sizeof(T) == 4;
     a.in[0] = s->r;
@@ -1014,6 +1038,11 @@ static int launch_fwd_t(Solver* s, long long* trace) {
     a.in[4] = s->qv;
     a.out[0] = sp ? (void*)s->y32 : s->yv;
     a.out[1] = s->r;
+    return a;
+}
+template <int MODE, typename T>
+static int launch_fwd_t(Solver* s, long long* trace) {
+    StreamArgs a = fwd_args<MODE, T>(s);
     a.trace = trace;
     constexpr int NG = MODE == 1 ? 5 : 4;
     if (s->level == 3) return launch_stream<SW_FWD, MODE, 3, NG, T>(s, a);
@@ -1026,7 +1055,7 @@ static int launch_fwd(Solver* s, long long* trace = nullptr) {
 }
 
 template <int MODE, typename T>
-static int launch_bwd_t(Solver* s, long long* trace) {
+static StreamArgs bwd_args(Solver* s) {
     StreamArgs a = stream_args(s, SW_BWD);
     const bool sp = sizeof(T) == 4;
     a.in[0] = sp ? (const void*)s->y32 : s->yv;
@@ -1036,6 +1065,11 @@ static int launch_bwd_t(Solver* s, long long* trace) {
     a.in[4] = s->r;
     a.in[5] = MODE == 1 ? s->qv : s->z;  // the reference's _z at non-FLUID cells (levels 4-5 dot)
     a.out[0] = s->z;
+    return a;
+}
+template <int MODE, typename T>
+static int launch_bwd_t(Solver* s, long long* trace) {
+    StreamArgs a = bwd_args<MODE, T>(s);
     a.trace = trace;
     if (s->level == 3) return launch_stream<SW_BWD, MODE, 3, 5, T>(s, a);
     if (s->level <= 5) return launch_stream<SW_BWD, MODE, 4, 6, T>(s, a);
@@ -1224,6 +1258,266 @@ int k_project(Solver* s, int limit, ifl_solve_stats* stats) {
             printf("Exceeded budget of %d iterations, maximum error was %f\n", limit, h.max_err);
         else if (h.status == IFL_SOLVE_INITIAL_SMALL && s->level >= 6)
             printf("Initial guess sufficiently small\n");
+    }
+    return IFL_OK;
+}
+
+// ---------------------------------------------------------------- batched solves
+// project() of several independent solvers of one level together: each phase
+// of a PCG iteration is one launch for all of them, its blocks dealt from a
+// table of (solver, block) pairs. Sweep strips are listed upstream first
+// inside each solver (ascending for the forward sweep, descending for the
+// backward one), so a strip's upstream neighbour always has the lower block
+// id: it is dispatched first, and the strips need not all be resident at
+// once however many solvers the batch holds. Solvers that have converged
+// return at once from every launch.
+struct BatchInst {
+    VecArgs v;
+    int nbv;      // vector blocks of 256 cells
+    int nstrips;  // strips, = the backward sweep's partials
+    double tol;
+    int limit;
+    int pad;
+};
+
+struct BatchDev {
+    int n = 0, nvblk = 0, nsblk = 0;
+    size_t shmem_f = 0, shmem_b = 0;
+    BatchInst* inst = nullptr;    // n
+    StreamArgs* fwd = nullptr;    // n
+    StreamArgs* bwd = nullptr;    // n
+    int2* vmap = nullptr;         // nvblk
+    int2* fmap = nullptr;         // nsblk
+    int2* bmap = nullptr;         // nsblk
+    PcgState* states = nullptr;   // n gathered states
+    PcgState* states_host = nullptr;
+    PcgState** st_ptrs = nullptr; // n
+    std::vector<Solver*> solvers; // the list the tables were built for
+};
+
+void batch_dev_free(BatchDev* d) {
+    if (!d) return;
+    auto F = [](void* p) { if (p) (void)hipFree(p); };
+    F(d->inst); F(d->fwd); F(d->bwd); F(d->vmap); F(d->fmap); F(d->bmap); F(d->states); F(d->st_ptrs);
+    if (d->states_host) (void)hipHostFree(d->states_host);
+    delete d;
+}
+
+namespace {
+
+__global__ __launch_bounds__(256) void batch_vec_update_kernel(const BatchInst* t, const int2* map) {
+    const int2 m = map[blockIdx.x];
+    vec_update_body(t[m.x].v, m.y, 0, 1);
+}
+
+__global__ __launch_bounds__(256) void batch_matvec_kernel(const BatchInst* t, const int2* map) {
+    const int2 m = map[blockIdx.x];
+    matvec_body<1>(t[m.x].v, m.y, 1);
+}
+
+__global__ __launch_bounds__(256) void batch_fin_alpha_kernel(const BatchInst* t) {
+    fin_alpha_body(t[blockIdx.x].v, t[blockIdx.x].nbv, 0);
+}
+
+__global__ void batch_fin_check_kernel(const BatchInst* t, int n) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) fin_check_body(t[i].v.st, t[i].tol);
+}
+
+__global__ __launch_bounds__(256) void batch_fin_sigma_kernel(const BatchInst* t) {
+    const BatchInst& b = t[blockIdx.x];
+    fin_sigma_body(b.v, b.nstrips, 0, 0, b.tol, b.limit);
+}
+
+template <int KIND, int LV, int NG>
+__global__ __launch_bounds__(64 * kWaves) void batch_sweep_kernel(const StreamArgs* t, const int2* map) {
+    const int2 m = map[blockIdx.x];
+    const StreamArgs a = t[m.x];
+    sweep_body<KIND, 1, LV, NG, double>(a, m.y);
+}
+
+__global__ void batch_gather_kernel(PcgState* const* st, int n, PcgState* out) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) out[i] = *st[i];
+}
+
+template <typename X>
+int upload(Solver* s, X** dst, const std::vector<X>& v) {
+    IFL_HIP(hipMalloc((void**)dst, std::max<size_t>(1, v.size()) * sizeof(X)));
+    if (!v.empty()) IFL_HIP(hipMemcpy(*dst, v.data(), v.size() * sizeof(X), hipMemcpyHostToDevice));
+    return IFL_OK;
+}
+
+template <int LV, int NGF, int NGB>
+int batch_attrs(Solver* s, BatchDev* d) {
+    static std::atomic<bool> set{false};
+    if (!set.load(std::memory_order_acquire)) {
+        IFL_HIP(hipFuncSetAttribute((const void*)batch_sweep_kernel<SW_FWD, LV, NGF>,
+                                    hipFuncAttributeMaxDynamicSharedMemorySize, (int)d->shmem_f));
+        IFL_HIP(hipFuncSetAttribute((const void*)batch_sweep_kernel<SW_BWD, LV, NGB>,
+                                    hipFuncAttributeMaxDynamicSharedMemorySize, (int)d->shmem_b));
+        set.store(true, std::memory_order_release);
+    }
+    return IFL_OK;
+}
+
+template <int LV, int NGF, int NGB>
+int batch_build(Solver* const* ss, int n, BatchDev* d) {
+    Solver* s0 = ss[0];
+    std::vector<BatchInst> inst(n);
+    std::vector<StreamArgs> fwd(n), bwd(n);
+    std::vector<int2> vmap, fmap, bmap;
+    std::vector<PcgState*> sts(n);
+    for (int i = 0; i < n; i++) {
+        Solver* s = ss[i];
+        BatchInst& b = inst[i];
+        b.v = vec_args(s);
+        b.nbv = grid_blocks(skew_n(s), 256);
+        b.nstrips = s->sk.nstrips;
+        b.tol = s->cfg.tol;
+        b.limit = s->cfg.pcg_limit;
+        b.pad = 0;
+        fwd[i] = fwd_args<1, double>(s);
+        bwd[i] = bwd_args<1, double>(s);
+        size_t shf = 0, shb = 0;
+        IFL_TRY((ring_setup<SW_FWD, 1, LV, NGF, double>(s, fwd[i], &shf)));
+        IFL_TRY((ring_setup<SW_BWD, 1, LV, NGB, double>(s, bwd[i], &shb)));
+        d->shmem_f = shf;
+        d->shmem_b = shb;
+        for (int k = 0; k < b.nbv; k++) vmap.push_back(make_int2(i, k));
+        for (int j = 0; j < b.nstrips; j++) fmap.push_back(make_int2(i, j));
+        for (int j = b.nstrips - 1; j >= 0; j--) bmap.push_back(make_int2(i, j));
+        sts[i] = s->st;
+    }
+    Solver* s = s0;  // error reporting for the uploads
+    d->n = n;
+    d->nvblk = (int)vmap.size();
+    d->nsblk = (int)fmap.size();
+    IFL_TRY(upload(s, &d->inst, inst));
+    IFL_TRY(upload(s, &d->fwd, fwd));
+    IFL_TRY(upload(s, &d->bwd, bwd));
+    IFL_TRY(upload(s, &d->vmap, vmap));
+    IFL_TRY(upload(s, &d->fmap, fmap));
+    IFL_TRY(upload(s, &d->bmap, bmap));
+    IFL_TRY(upload(s, &d->st_ptrs, sts));
+    IFL_HIP(hipMalloc((void**)&d->states, n * sizeof(PcgState)));
+    IFL_HIP(hipHostMalloc((void**)&d->states_host, n * sizeof(PcgState), hipHostMallocDefault));
+    d->solvers.assign(ss, ss + n);
+    return batch_attrs<LV, NGF, NGB>(s, d);
+}
+
+template <int LV, int NGF, int NGB>
+int batch_iteration(Solver* s, BatchDev* d, hipStream_t stream, int it) {
+    if (it > 0) {
+        batch_vec_update_kernel<<<d->nvblk, 256, 0, stream>>>(d->inst, d->vmap);
+        IFL_LAUNCH_CHECK();
+    }
+    batch_matvec_kernel<<<d->nvblk, 256, 0, stream>>>(d->inst, d->vmap);
+    IFL_LAUNCH_CHECK();
+    batch_fin_alpha_kernel<<<d->n, 256, 0, stream>>>(d->inst);
+    IFL_LAUNCH_CHECK();
+    batch_sweep_kernel<SW_FWD, LV, NGF><<<d->nsblk, 64 * kWaves, d->shmem_f, stream>>>(d->fwd, d->fmap);
+    IFL_LAUNCH_CHECK();
+    batch_fin_check_kernel<<<grid_blocks(d->n, 256), 256, 0, stream>>>(d->inst, d->n);
+    IFL_LAUNCH_CHECK();
+    batch_sweep_kernel<SW_BWD, LV, NGB><<<d->nsblk, 64 * kWaves, d->shmem_b, stream>>>(d->bwd, d->bmap);
+    IFL_LAUNCH_CHECK();
+    batch_fin_sigma_kernel<<<d->n, 256, 0, stream>>>(d->inst);
+    IFL_LAUNCH_CHECK();
+    return IFL_OK;
+}
+
+int batch_gather(Solver* s, BatchDev* d, hipStream_t stream) {
+    batch_gather_kernel<<<grid_blocks(d->n, 256), 256, 0, stream>>>(d->st_ptrs, d->n, d->states);
+    IFL_LAUNCH_CHECK();
+    IFL_HIP(hipMemcpyAsync(d->states_host, d->states, d->n * sizeof(PcgState), hipMemcpyDeviceToHost, stream));
+    IFL_HIP(hipStreamSynchronize(stream));
+    return IFL_OK;
+}
+
+}  // namespace
+
+// The batched project(): the entry (7:854-864) and the exit tails per solver,
+// exactly as k_project does them, and the iterations in between batched.
+int k_project_batch(Solver* const* ss, int n, hipStream_t stream, BatchDev** dev, ifl_solve_stats* stats) {
+    Solver* s0 = ss[0];
+    const int lv = s0->level;
+    if (lv == 8 || s0->cfg.precision != IFL_PREC_FP64 || s0->cfg.dot_order != IFL_DOT_TREE) {
+        s0->err = "batched solves take levels 3-7, the FP64 form and tree-ordered dots";
+        return IFL_ERR_UNSUPPORTED;
+    }
+    BatchDev* d = *dev;
+    if (!d || d->solvers.size() != (size_t)n || !std::equal(ss, ss + n, d->solvers.begin())) {
+        batch_dev_free(d);
+        d = *dev = new BatchDev();
+        Solver* s = s0;
+        IFL_HIP(hipSetDevice(s0->device));
+        int rc = lv == 3 ? batch_build<3, 5, 5>(ss, n, d) : lv <= 5 ? batch_build<4, 5, 6>(ss, n, d)
+                                                                     : batch_build<6, 5, 5>(ss, n, d);
+        if (rc != IFL_OK) return rc;
+    }
+    int limit = 0;
+    for (int i = 0; i < n; i++) {
+        Solver* s = ss[i];
+        VecArgs a = vec_args(s);
+        long nn = skew_n(s);
+        state_reset_kernel<<<1, 1, 0, stream>>>(s->st);
+        IFL_LAUNCH_CHECK();
+        IFL_HIP(hipMemsetAsync(s->p, 0, nn * sizeof(double), stream));  // 7:854
+        IFL_TRY(launch_fwd<0>(s));                                      // 7:855
+        IFL_TRY(launch_bwd<0>(s));
+        IFL_HIP(hipMemcpyAsync(s->s, s->z, nn * sizeof(double), hipMemcpyDeviceToDevice, stream));  // 7:856
+        fin_sigma_kernel<<<1, 256, 0, stream>>>(a, s->sk.nstrips, 0, 1, s->cfg.tol, s->cfg.pcg_limit);
+        IFL_LAUNCH_CHECK();
+        limit = std::max(limit, s->cfg.pcg_limit);
+    }
+    Solver* s = s0;
+    int launched = 0, batch = 4;
+    while (true) {
+        IFL_TRY(batch_gather(s, d, stream));
+        bool all = true;
+        for (int i = 0; i < n; i++) {
+            if (d->states_host[i].timeout) {
+                ss[i]->err = "PCG sweep hand-off timed out";
+                s0->err = ss[i]->err;
+                return IFL_ERR_DEVICE;
+            }
+            all = all && d->states_host[i].done;
+        }
+        if (all || launched >= limit) break;
+        const int nb = std::min(batch, limit - launched);
+        for (int k = 0; k < nb; k++) {
+            int rc = lv == 3 ? batch_iteration<3, 5, 5>(s, d, stream, launched + k)
+                   : lv <= 5 ? batch_iteration<4, 5, 6>(s, d, stream, launched + k)
+                             : batch_iteration<6, 5, 5>(s, d, stream, launched + k);
+            if (rc != IFL_OK) return rc;
+        }
+        launched += nb;
+        batch = std::min(batch * 2, 64);
+    }
+    for (int i = 0; i < n; i++) {
+        Solver* si = ss[i];
+        const PcgState& h = d->states_host[i];
+        VecArgs a = vec_args(si);
+        long nn = skew_n(si);
+        s = si;
+        if (h.status == IFL_SOLVE_CONVERGED) {
+            vec_update_kernel<<<grid_blocks(nn, 256), 256, 0, stream>>>(a, 2, 0);  // p += alpha*s (7:869)
+            IFL_LAUNCH_CHECK();
+            IFL_HIP(hipMemcpyAsync(si->z, si->qv, nn * sizeof(double), hipMemcpyDeviceToDevice, stream));
+        } else if (h.status == IFL_SOLVE_BUDGET && si->cfg.pcg_limit > 0) {
+            vec_update_kernel<<<grid_blocks(nn, 256), 256, 0, stream>>>(a, 0, 0);
+            IFL_LAUNCH_CHECK();
+            if (a.pm) {
+                z_nonfluid_kernel<<<grid_blocks(nn, 256), 256, 0, stream>>>(a);
+                IFL_LAUNCH_CHECK();
+            }
+        }
+        if (stats) {
+            stats[i].status = h.status;
+            stats[i].iterations = h.printed_iter;
+            stats[i].max_error = h.max_err;
+        }
     }
     return IFL_OK;
 }

@@ -101,6 +101,9 @@ def _load():
     lib.ifl_init_particles.argtypes = [vp]
     lib.ifl_max_timestep.argtypes = [vp, ctypes.POINTER(ctypes.c_double)]
     lib.ifl_to_image.argtypes = [vp, vp, ctypes.c_size_t, ctypes.c_int]
+    lib.ifl_batch_create.argtypes = [ctypes.POINTER(vp), ctypes.c_int, ctypes.POINTER(vp)]
+    lib.ifl_batch_update.argtypes = [vp, ctypes.c_double, ctypes.POINTER(StepStats)]
+    lib.ifl_batch_destroy.argtypes = [vp]
     lib.ifl_write_png.argtypes = [ctypes.c_char_p, vp, ctypes.c_int, ctypes.c_int]
     lib.ifl_frand_state.restype = ctypes.c_uint32
     lib.ifl_frand_state.argtypes = []
@@ -313,3 +316,36 @@ def write_png(path, rgba):
     rc = lib.ifl_write_png(os.fsencode(path), a.ctypes.data, w, h)
     if rc != 0:
         raise IflError(f"ifl_write_png failed rc={rc}: {lib.ifl_last_error(None).decode()}")
+
+
+class Batch:
+    """Independent solvers of one level stepped together (ifl_batch_*): each
+    PCG iteration of their solves is one launch per phase for all of them."""
+
+    def __init__(self, solvers):
+        self.solvers = list(solvers)
+        arr = (ctypes.c_void_p * len(self.solvers))(*[s._h.value for s in self.solvers])
+        self._h = ctypes.c_void_p()
+        rc = lib.ifl_batch_create(arr, len(self.solvers), ctypes.byref(self._h))
+        if rc != 0:
+            raise IflError(f"ifl_batch_create failed rc={rc}: {lib.ifl_last_error(None).decode()}")
+
+    def update(self, dt):
+        for s in self.solvers:
+            s._push_bodies()
+        st = (StepStats * len(self.solvers))()
+        rc = lib.ifl_batch_update(self._h, dt, st)
+        if rc != 0:
+            raise IflError(f"ifl_batch_update failed rc={rc}: {lib.ifl_last_error(None).decode()}")
+        return list(st)
+
+    def close(self):
+        if self._h:
+            lib.ifl_batch_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

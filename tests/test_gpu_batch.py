@@ -50,3 +50,51 @@ def test_concurrent_instances_bit_exact():
         print(f"instance {k} {inst['w']}x{inst['h']}: bit-exact after {steps} steps", flush=True)
         orc.close()
         gpus[k].close()
+
+
+@pytest.mark.parametrize("level", [4, 7])
+def test_fused_batch_equals_single_solvers(level):
+    """ifl_batch_update runs every instance's stages and batches the PCG
+    iterations of their solves; each instance must end exactly where the
+    same instance stepped on its own ends (same kernels, same reduction
+    order per instance), while the instances converge after different
+    iteration counts."""
+    import numpy as np
+    import ifl_fluids as F
+    import ifl_scenes as S
+    insts = [i for i in S.config4_instances(64) if max(i["w"], i["h"]) <= 192][:6]
+    params = (0.1,) if level == 4 else (0.1, 1.0, 0.01)
+    inflow = lambda i: i["inflows"][0] if level == 4 else i["inflows"][0][:5] + (600.0, 0.0, 0.0)
+
+    def make(i):
+        return F.FluidSolver(level, i["w"], i["h"], params, [F.SolidBox(*b[1:]) for b in i["bodies"]])
+    steps = 3
+    batched = [make(i) for i in insts]
+    batch = F.Batch(batched)
+    iters_b, stat_b = [], []
+    for _ in range(steps):
+        for s, i in zip(batched, insts):
+            s.add_inflow(*inflow(i))
+        st = batch.update(insts[0]["dt"])
+        iters_b.append([(x.heat.iterations, x.pressure.iterations) for x in st])
+        stat_b.append([(x.heat.status, x.pressure.status) for x in st])
+    batch.close()
+    fields = ol.state_fields(level)
+    for k, i in enumerate(insts):
+        single = make(i)
+        its, stat = [], []
+        for _ in range(steps):
+            single.add_inflow(*inflow(i))
+            st = single.update(i["dt"])
+            its.append((st.heat.iterations, st.pressure.iterations))
+            stat.append((st.heat.status, st.pressure.status))
+        assert its == [row[k] for row in iters_b]
+        assert stat == [row[k] for row in stat_b]
+        bad = compare_fields(single, batched[k], fields, where=f"instance {k} {i['w']}x{i['h']}")
+        assert not bad, "\n".join(bad[:6])
+        single.close()
+        batched[k].close()
+    counts = sorted(set(row[k][1] for row in iters_b for k in range(len(insts))))
+    print(f"level {level}: (heat, pressure) iterations per step and instance {iters_b}; statuses {stat_b}",
+          flush=True)
+    assert len(counts) > 1  # the instances really converge at different iterations
