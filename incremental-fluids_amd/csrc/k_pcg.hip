@@ -473,7 +473,7 @@ IFL_DEV void compute_wave(const StreamArgs& a, char* ring, RingHdr* hdr, const d
                 const T tt = t1 - x(2) * zn;
                 const T zz = tt * x(3);
                 zs = f ? zz : (T)0;
-                out[e] = zz;
+                out[e] = zs;
             } else {
                 // buildPreconditioner 7:744-767 (T is double)
                 const double Ns = lane_shr1((double)Pn, (double)ek);
@@ -519,7 +519,7 @@ IFL_DEV void compute_wave(const StreamArgs& a, char* ring, RingHdr* hdr, const d
     }
 }
 
-// Wave 2: drains finished blocks.
+// The writer wave: drains finished blocks.
 template <int KIND, int MODE, int LV, int NG, typename T>
 IFL_DEV void writer_wave(const StreamArgs& a, char* ring, RingHdr* hdr, int j, int lane, double* eout) {
     typedef SweepShape<KIND, MODE, NG, T> S;
@@ -569,6 +569,10 @@ IFL_DEV void writer_wave(const StreamArgs& a, char* ring, RingHdr* hdr, int j, i
                 st_relaxed(eout + x, as_bits(v));
             }
         }
+        // Per step: loads first and selects instead of branches, so that the
+        // loop is straight-line code with no masked stores (the wave 0
+        // recurrence on the neighbouring SIMD measurably slows down while
+        // this wave runs branchy code or masked stores).
 #pragma unroll
         for (int k = 0; k < kBlk; k++) {
             const int t = b * kBlk + k;
@@ -577,15 +581,30 @@ IFL_DEV void writer_wave(const StreamArgs& a, char* ring, RingHdr* hdr, int j, i
             const int x = c - lane;
             const bool valid = row_ok && x >= 0 && x < W;
             const bool f = lane_on(M[c - col0]);
+            const bool take = VM ? f : valid;
             if (KIND == SW_FWD) {
                 // infinityNorm(_r) 7:871 (masked in 6+) of the residual the sweep used
-                if (VM ? f : valid) rmax = smax(rmax, fabs(((const double*)in(0))[e]));
+                const double rv = ((const double*)in(0))[e];
+                const double m = smax(rmax, fabs(rv));
+                rmax = take ? m : rmax;
             } else if (KIND == SW_BWD) {
-                // store _z at FLUID cells; dotProduct(_z, _r) 7:880 (unmasked in 3-5)
+                // _z (wave 0 left +0 at non-FLUID cells); dotProduct(_z, _r)
+                // 7:880 (unmasked in 3-5, where the reference's _z is A s at
+                // non-FLUID cells)
                 const double zz = (double)out[e];
-                if (f) ((double*)a.out[0])[gblk + e] = zz;
+                if (MODE == 0) {
+                    // the stage form keeps _z at non-FLUID cells as it was
+                    if (f) ((double*)a.out[0])[gblk + e] = zz;
+                } else if (sizeof(T) != 8) {
+                    // inside the PCG loop _z at non-FLUID cells is not read
+                    // (levels 4-5 take A s from q; the exit restores it), so
+                    // every cell is stored; the mixed form widens on the way
+                    ((double*)a.out[0])[gblk + e] = zz;
+                }
+                const double rv = ((const double*)in(4))[e];
                 const double zdot = f ? zz : (AUX ? ((const double*)in(5))[e] : 0.0);
-                if (VM ? f : valid) acc += zdot * ((const double*)in(4))[e];
+                const double na = acc + zdot * rv;
+                acc = take ? na : acc;
             } else {
                 // the factor at FLUID cells; cx = aPlusX*precon, cy likewise, +0 elsewhere
                 const double pcv = (double)out[e];
@@ -612,6 +631,8 @@ IFL_DEV void writer_wave(const StreamArgs& a, char* ring, RingHdr* hdr, int j, i
         if (KIND == SW_FWD) {
             store_block(a.out[0], (const char*)out, sizeof(T));
             if (MODE == 1) store_block(a.out[1], in(0), 8);
+        } else if (KIND == SW_BWD) {
+            if (MODE == 1 && sizeof(T) == 8) store_block(a.out[0], (const char*)out, 8);
         } else if (KIND == SW_PRECON) {
             store_block(a.out[1], in(1), 8);
             store_block(a.out[2], in(2), 8);
