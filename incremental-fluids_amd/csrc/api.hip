@@ -320,11 +320,11 @@ int solve(Solver* s, ifl_solve_stats* st) {
 
 void flip_q(Solver* s, int qi) { std::swap(s->q[qi].src, s->q[qi].dst); }
 
-int read_border_flag(Solver* s) {
-    int h = 0;
-    IFL_HIP(hipMemcpyAsync(&h, s->front_cnt + 1, sizeof(int), hipMemcpyDeviceToHost, s->stream));
-    IFL_HIP(hipStreamSynchronize(s->stream));
-    s->border_solid |= h;
+// The border flag of the last extrapolation goes to step_flags[1] on the
+// device and reaches the host with the end-of-step read (no wait here).
+int defer_border_flag(Solver* s) {
+    IFL_HIP(hipMemcpyAsync(s->step_flags + 1, s->front_cnt + 1, sizeof(int), hipMemcpyDeviceToDevice, s->stream));
+    s->border_pending = true;
     return IFL_OK;
 }
 
@@ -350,6 +350,7 @@ int update_part(Solver* s, int after, double dt, int* next) {
     *next = SOLVE_NONE;
     if (after == SOLVE_NONE) {
         s->border_solid = 0;
+        s->border_pending = false;
         if (lv == 3) {
             IFL_TRY(k_build_rhs(s));
             IFL_TRY(k_build_pressure(s, dt));
@@ -406,7 +407,7 @@ int update_part(Solver* s, int after, double dt, int* next) {
     IFL_TRY(extrapolate(s, IFL_Q_D));
     IFL_TRY(extrapolate(s, IFL_Q_U));
     IFL_TRY(extrapolate(s, IFL_Q_V));
-    if (lv < 8) IFL_TRY(read_border_flag(s));
+    if (lv < 8) IFL_TRY(defer_border_flag(s));
     IFL_TRY(k_set_boundary(s));
     if (lv == 8) {
         // 8:1393-1412: FLIP update of the particles, then their advection
@@ -430,8 +431,10 @@ int update_part(Solver* s, int after, double dt, int* next) {
 // the end of a step: the non-finite check, timing collection, statistics
 int finish_step(Solver* s, const ifl_solve_stats& heat, const ifl_solve_stats& press, ifl_step_stats* stats) {
     IFL_TRY(k_nonfinite(s, s->step_flags));
-    IFL_HIP(hipMemcpyAsync(s->step_flags_host, s->step_flags, sizeof(int), hipMemcpyDeviceToHost, s->stream));
+    IFL_HIP(hipMemcpyAsync(s->step_flags_host, s->step_flags, 2 * sizeof(int), hipMemcpyDeviceToHost, s->stream));
     IFL_HIP(hipStreamSynchronize(s->stream));
+    if (s->border_pending) s->border_solid |= s->step_flags_host[1];
+    s->border_pending = false;
     IFL_TRY(prof_collect(s, nullptr));
     if (stats) {
         stats->nonfinite = s->step_flags_host[0];
@@ -821,13 +824,10 @@ int ifl_batch_update(ifl_batch* b, double dt, ifl_step_stats* stats) {
     if (!b) return IFL_ERR_ARG;
     const int n = (int)b->solvers.size();
     if (hipSetDevice(b->device) != hipSuccess) return IFL_ERR_HIP;
-    // every stage of every solver goes to the batch's stream, in order
-    std::vector<hipStream_t> own(n);
-    for (int i = 0; i < n; i++) {
-        own[i] = b->solvers[i]->stream;
-        if (hipStreamSynchronize(own[i]) != hipSuccess) return IFL_ERR_HIP;
-    }
-    for (int i = 0; i < n; i++) b->solvers[i]->stream = b->stream;
+    // The per-solver stages run on each solver's own stream, so that the
+    // small kernels of different solvers overlap; the batched iterations run
+    // on the batch's stream, joined to the solver streams by events
+    // (k_project_batch).
     const ifl_solve_stats none = {IFL_SOLVE_NOT_RUN, 0, 0.0};
     std::vector<ifl_solve_stats> heat(n, none), press(n, none);
     int rc = IFL_OK, after = SOLVE_NONE;
@@ -850,7 +850,6 @@ int ifl_batch_update(ifl_batch* b, double dt, ifl_step_stats* stats) {
         if (rc != IFL_OK) g_err = b->solvers[i]->err;
     }
     (void)hipStreamSynchronize(b->stream);
-    for (int i = 0; i < n; i++) b->solvers[i]->stream = own[i];
     return rc;
 }
 

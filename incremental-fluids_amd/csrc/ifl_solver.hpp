@@ -110,12 +110,13 @@ struct Solver {
     std::vector<hipEvent_t> ev_free;  // event pool for per-launch timing
     std::vector<Timed> timed;         // recorded, not yet collected
 
-    int* step_flags = nullptr;   // device: [0] non-finite field values after a step
+    int* step_flags = nullptr;   // device: [0] non-finite field values after a step, [1] border flag
     int* step_flags_host = nullptr;  // pinned mirror
     uint8_t* image_d = nullptr;  // toImage output, allocated on first use
     size_t image_bytes = 0;
 
     int border_solid = 0;
+    bool border_pending = false;  // step_flags[1] holds this step's border flag
     size_t lds_bytes = 65536;  // dynamic LDS available to one workgroup
     int num_cus = 1;           // compute units: the strips of one sweep launch that can be resident at once
 };
@@ -175,8 +176,9 @@ int k_matvec(Solver* s);             // z = A s, the stage form
 int k_dot(Solver* s, int which, double* out);  // 0: z.r  1: z.s
 int k_inf_norm(Solver* s, double* out);
 int k_project(Solver* s, int limit, ifl_solve_stats* stats);
-// project() of several solvers of one level at once, all on `stream`: one
-// launch per phase of each PCG iteration for all of them (stats: n entries)
+// project() of several solvers of one level at once: each solver's entry and
+// tail on its own stream, the iterations on `stream`, one launch per phase of
+// each PCG iteration for all of them (stats: n entries)
 struct BatchDev;
 int k_project_batch(Solver* const* ss, int n, hipStream_t stream, BatchDev** dev, ifl_solve_stats* stats);
 void batch_dev_free(BatchDev* d);

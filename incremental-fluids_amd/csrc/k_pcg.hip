@@ -1314,6 +1314,8 @@ struct BatchDev {
     PcgState* states_host = nullptr;
     PcgState** st_ptrs = nullptr; // n
     std::vector<Solver*> solvers; // the list the tables were built for
+    std::vector<hipEvent_t> ready;  // per solver: its entry into the solve is queued
+    hipEvent_t joined = nullptr;    // the batched iterations are queued
 };
 
 void batch_dev_free(BatchDev* d) {
@@ -1321,6 +1323,8 @@ void batch_dev_free(BatchDev* d) {
     auto F = [](void* p) { if (p) (void)hipFree(p); };
     F(d->inst); F(d->fwd); F(d->bwd); F(d->vmap); F(d->fmap); F(d->bmap); F(d->states); F(d->st_ptrs);
     if (d->states_host) (void)hipHostFree(d->states_host);
+    for (hipEvent_t e : d->ready) (void)hipEventDestroy(e);
+    if (d->joined) (void)hipEventDestroy(d->joined);
     delete d;
 }
 
@@ -1423,6 +1427,9 @@ int batch_build(Solver* const* ss, int n, BatchDev* d) {
     IFL_TRY(upload(s, &d->st_ptrs, sts));
     IFL_HIP(hipMalloc((void**)&d->states, n * sizeof(PcgState)));
     IFL_HIP(hipHostMalloc((void**)&d->states_host, n * sizeof(PcgState), hipHostMallocDefault));
+    d->ready.assign(n, nullptr);
+    for (int i = 0; i < n; i++) IFL_HIP(hipEventCreateWithFlags(&d->ready[i], hipEventDisableTiming));
+    IFL_HIP(hipEventCreateWithFlags(&d->joined, hipEventDisableTiming));
     d->solvers.assign(ss, ss + n);
     return batch_attrs<LV, NGF, NGB>(s, d);
 }
@@ -1477,19 +1484,23 @@ int k_project_batch(Solver* const* ss, int n, hipStream_t stream, BatchDev** dev
                                                                      : batch_build<6, 5, 5>(ss, n, d);
         if (rc != IFL_OK) return rc;
     }
+    // the entry of every solve on its solver's own stream (they overlap),
+    // then the batched iterations on `stream` once every entry is done
     int limit = 0;
     for (int i = 0; i < n; i++) {
         Solver* s = ss[i];
         VecArgs a = vec_args(s);
         long nn = skew_n(s);
-        state_reset_kernel<<<1, 1, 0, stream>>>(s->st);
+        state_reset_kernel<<<1, 1, 0, s->stream>>>(s->st);
         IFL_LAUNCH_CHECK();
-        IFL_HIP(hipMemsetAsync(s->p, 0, nn * sizeof(double), stream));  // 7:854
-        IFL_TRY(launch_fwd<0>(s));                                      // 7:855
+        IFL_HIP(hipMemsetAsync(s->p, 0, nn * sizeof(double), s->stream));  // 7:854
+        IFL_TRY(launch_fwd<0>(s));                                         // 7:855
         IFL_TRY(launch_bwd<0>(s));
-        IFL_HIP(hipMemcpyAsync(s->s, s->z, nn * sizeof(double), hipMemcpyDeviceToDevice, stream));  // 7:856
-        fin_sigma_kernel<<<1, 256, 0, stream>>>(a, s->sk.nstrips, 0, 1, s->cfg.tol, s->cfg.pcg_limit);
+        IFL_HIP(hipMemcpyAsync(s->s, s->z, nn * sizeof(double), hipMemcpyDeviceToDevice, s->stream));  // 7:856
+        fin_sigma_kernel<<<1, 256, 0, s->stream>>>(a, s->sk.nstrips, 0, 1, s->cfg.tol, s->cfg.pcg_limit);
         IFL_LAUNCH_CHECK();
+        IFL_HIP(hipEventRecord(d->ready[i], s->stream));
+        IFL_HIP(hipStreamWaitEvent(stream, d->ready[i], 0));
         limit = std::max(limit, s->cfg.pcg_limit);
     }
     Solver* s = s0;
@@ -1516,21 +1527,24 @@ int k_project_batch(Solver* const* ss, int n, hipStream_t stream, BatchDev** dev
         launched += nb;
         batch = std::min(batch * 2, 64);
     }
+    // the tails back on the solvers' streams, after the batched iterations
+    IFL_HIP(hipEventRecord(d->joined, stream));
     for (int i = 0; i < n; i++) {
         Solver* si = ss[i];
         const PcgState& h = d->states_host[i];
         VecArgs a = vec_args(si);
         long nn = skew_n(si);
         s = si;
+        IFL_HIP(hipStreamWaitEvent(si->stream, d->joined, 0));
         if (h.status == IFL_SOLVE_CONVERGED) {
-            vec_update_kernel<<<grid_blocks(nn, 256), 256, 0, stream>>>(a, 2, 0);  // p += alpha*s (7:869)
+            vec_update_kernel<<<grid_blocks(nn, 256), 256, 0, si->stream>>>(a, 2, 0);  // p += alpha*s (7:869)
             IFL_LAUNCH_CHECK();
-            IFL_HIP(hipMemcpyAsync(si->z, si->qv, nn * sizeof(double), hipMemcpyDeviceToDevice, stream));
+            IFL_HIP(hipMemcpyAsync(si->z, si->qv, nn * sizeof(double), hipMemcpyDeviceToDevice, si->stream));
         } else if (h.status == IFL_SOLVE_BUDGET && si->cfg.pcg_limit > 0) {
-            vec_update_kernel<<<grid_blocks(nn, 256), 256, 0, stream>>>(a, 0, 0);
+            vec_update_kernel<<<grid_blocks(nn, 256), 256, 0, si->stream>>>(a, 0, 0);
             IFL_LAUNCH_CHECK();
             if (a.pm) {
-                z_nonfluid_kernel<<<grid_blocks(nn, 256), 256, 0, stream>>>(a);
+                z_nonfluid_kernel<<<grid_blocks(nn, 256), 256, 0, si->stream>>>(a);
                 IFL_LAUNCH_CHECK();
             }
         }

@@ -176,3 +176,34 @@ def test_step_reports_nonfinite_fields():
     st = gpu.update(sc["dt"])
     assert st.nonfinite == 1
     gpu.close()
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("level", [4, 7])
+def test_step_reports_border_solid(level):
+    """ifl_step_stats.border_solid: 1 when a solid body covers cells on the
+    domain border (the reference's extrapolation then reads outside the
+    grid), 0 otherwise; read at the end of the step, also in the batched form."""
+    import math
+    import ifl_fluids as F
+    sc = ol.REF_SCENES[7 if level == 7 else 4]
+    inside = (F.BODY_BOX, 0.5, 0.4, 0.2, 0.1, 0.0, 0.0, 0.0, 0.0)
+    touching = (F.BODY_BOX, 0.02, 0.4, 0.2, 0.1, math.pi * 0.25, 0.0, 0.0, 0.0)
+    got = []
+    for body in (inside, touching):
+        gpu = make_product(level, 64, 48, sc["params"], [body])
+        gpu.add_inflow(*sc["inflow"])
+        got.append(gpu.update(sc["dt"]).border_solid)
+        gpu.close()
+    assert got == [0, 1]
+    # the same through the batched form, two solvers side by side
+    objs = [[F.SolidBox(*inside[1:])], [F.SolidBox(*touching[1:])]]
+    solvers = [F.FluidSolver(level, 64, 48, sc["params"], o) for o in objs]
+    batch = F.Batch(solvers)
+    for s in solvers:
+        s.add_inflow(*sc["inflow"])
+    stats = batch.update(sc["dt"])
+    assert [st.border_solid for st in stats] == [0, 1]
+    batch.close()
+    for s in solvers:
+        s.close()
