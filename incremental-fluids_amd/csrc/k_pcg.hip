@@ -262,29 +262,36 @@ IFL_DEV void loader_wave(const StreamArgs& a, char* ring, RingHdr* hdr, int k, i
     const double alpha_neg = UPD ? -a.st->alpha : 0.0;
     const const_u64* fm = strip_masks(a, j);
     vec4i b0[NL], b1[NL], b2[NL];
-    auto issue = [&](int i, vec4i* B) __attribute__((always_inline)) {
-        const long e0 = sbase + (long)block_col0<KIND>(k + i * kLoaders, cols) * kStrip;
+    // the FLUID masks the residual update needs (levels 6+), loaded with the
+    // block so that their latency is hidden like the block's
+    constexpr bool MASKED = UPD && LV >= 6;
+    uint64_t w0[4], w1[4], w2[4];
+    auto issue = [&](int i, vec4i* B, uint64_t* WM) __attribute__((always_inline)) {
+        const int col0 = block_col0<KIND>(k + i * kLoaders, cols);
+        const long e0 = sbase + (long)col0 * kStrip;
 #pragma unroll
         for (int ai = 0; ai < NG; ai++) {
             const char* p = (const char*)a.in[ai] + e0 * S::es(ai) + lane * 16;
 #pragma unroll
             for (int m = 0; m < S::rb(ai) / 1024; m++) B[S::lo(ai) + m] = __builtin_nontemporal_load((const vec4i*)(p + m * 1024));
         }
+        if (MASKED) {
+            // element 2*lane + c of run m is step 2m + lane/32, row 2*(lane%32) + c
+#pragma unroll
+            for (int m = 0; m < 4; m++) WM[m] = fm[col0 + 2 * m + (lane >> 5)];
+        }
     };
     // land block i (from B) in its slot, then reuse B for block i + R
-    auto stage = [&](int i, vec4i* B) __attribute__((always_inline)) -> bool {
+    auto stage = [&](int i, vec4i* B, uint64_t* WM) __attribute__((always_inline)) -> bool {
         if (i >= nmine) return true;
         const int b = k + i * kLoaders;
         if (UPD) {
-            const int col0 = block_col0<KIND>(b, cols);
 #pragma unroll
             for (int m = 0; m < 4; m++) {
                 const vec2d r = __builtin_bit_cast(vec2d, B[m]);
                 vec2d rn = r + __builtin_bit_cast(vec2d, B[S::lo(4) + m]) * alpha_neg;
-                if (LV >= 6) {
-                    // element 2*lane + c of run m is step 2m + lane/32, row 2*(lane%32) + c
-                    const uint64_t w = lane < 32 ? fm[col0 + 2 * m] : fm[col0 + 2 * m + 1];
-                    const unsigned bits = (unsigned)(w >> (2 * (lane & 31)));
+                if (MASKED) {
+                    const unsigned bits = (unsigned)(WM[m] >> (2 * (lane & 31)));
                     rn.x = (bits & 1u) ? rn.x : r.x;
                     rn.y = (bits & 2u) ? rn.y : r.y;
                 }
@@ -299,16 +306,16 @@ IFL_DEV void loader_wave(const StreamArgs& a, char* ring, RingHdr* hdr, int k, i
             for (int m = 0; m < S::rb(ai) / 1024; m++)
                 *(vec4i*)(slot + S::ro(ai) + m * 1024 + lane * 16) = B[S::lo(ai) + m];
         lds_st(&hdr->ready[k], b + 1);
-        if (i + R < nmine) issue(i + R, B);
+        if (i + R < nmine) issue(i + R, B, WM);
         return true;
     };
-    if (nmine > 0) issue(0, b0);
-    if (R > 1 && nmine > 1) issue(1, b1);
-    if (R > 2 && nmine > 2) issue(2, b2);
+    if (nmine > 0) issue(0, b0, w0);
+    if (R > 1 && nmine > 1) issue(1, b1, w1);
+    if (R > 2 && nmine > 2) issue(2, b2, w2);
     for (int i = 0; i < nmine; i += R) {
-        if (!stage(i, b0)) return;
-        if (R > 1 && !stage(i + 1, b1)) return;
-        if (R > 2 && !stage(i + 2, b2)) return;
+        if (!stage(i, b0, w0)) return;
+        if (R > 1 && !stage(i + 1, b1, w1)) return;
+        if (R > 2 && !stage(i + 2, b2, w2)) return;
     }
 }
 
