@@ -503,6 +503,13 @@ IFL_DEV void compute_wave(const StreamArgs& a, char* ring, RingHdr* hdr, const d
         }
     };
 
+    // the strip's first masks, fetched while it waits for its first edges
+    // (otherwise the first half would wait for them from HBM)
+    {
+        const int c0 = KIND == SW_BWD ? step_col<KIND>(kHalf - 1, cols) : step_col<KIND>(0, cols);
+        const vec4u m = *(const const_u64x4*)(fm + c0);
+        asm volatile("" : : "s"(m.x));
+    }
     if (!ready(0, kHalf)) return;
     const char* slot = ring;  // block 0 is in slot 0
     load_half(slot, 0, XA, QA, EA, MA);
@@ -547,20 +554,28 @@ IFL_DEV void writer_wave(const StreamArgs& a, char* ring, RingHdr* hdr, int j, i
         const T* out = (const T*)(slot + S::OUT);
         const int col0 = block_col0<KIND>(b, cols);
         const long gblk = sbase + (long)col0 * kStrip;  // first element of the block in HBM
-        // the block's 8 masks in one scalar load, before any of them is used
+        // the block's 8 masks in one scalar load; the sweeps' boundary rows
+        // need none, so there they are loaded after it (scalar loads return
+        // out of order: the row's LDS reads would wait for them)
         uint64_t M[kBlk];
+        if (KIND == SW_PRECON) {
 #pragma unroll
-        for (int k = 0; k < kBlk; k++) M[k] = fm[col0 + k];
+            for (int k = 0; k < kBlk; k++) M[k] = fm[col0 + k];
+        }
         // first the boundary row for the neighbouring strip (it waits for it),
         // one step per lane: the same operations on the same operands as
         // wave 0 performed
         if (eout) {
-            // lane i < 8 takes step i of the block: its column's mask
+            // lane i < 8 takes step i of the block (the factor build: its
+            // column's mask)
             const int t = b * kBlk + (lane & (kBlk - 1));
             const int c = step_col<KIND>(t, cols);
-            uint64_t mc = M[0];
+            uint64_t mc = 0;
+            if (KIND == SW_PRECON) {
+                mc = M[0];
 #pragma unroll
-            for (int k = 1; k < kBlk; k++) mc = c - col0 == k ? M[k] : mc;
+                for (int k = 1; k < kBlk; k++) mc = c - col0 == k ? M[k] : mc;
+            }
             const int e = step_in_block<KIND>(lane & (kBlk - 1)) * kStrip + (KIND == SW_BWD ? 0 : kStrip - 1);
             const int x = KIND == SW_BWD ? c : c - (kStrip - 1);
             if (lane < kBlk && x >= 0 && x < W) {
@@ -569,7 +584,7 @@ IFL_DEV void writer_wave(const StreamArgs& a, char* ring, RingHdr* hdr, int j, i
                     const T pn = ((const T*)in(2))[e] * out[e];  // Pn = cy * zm of lane 63
                     v = (double)pn;
                 } else if (KIND == SW_BWD) {
-                    v = (mc & 1ull) ? (double)out[e] : 0.0;  // zs of lane 0
+                    v = (double)out[e];  // zs of lane 0 (+0 at a non-FLUID cell)
                 } else {
                     const double px = ((const double*)in(1))[e] * (double)out[e];
                     const double py = ((const double*)in(2))[e] * (double)out[e];
@@ -577,6 +592,10 @@ IFL_DEV void writer_wave(const StreamArgs& a, char* ring, RingHdr* hdr, int j, i
                 }
                 st_relaxed(eout + x, as_bits(v));
             }
+        }
+        if (KIND != SW_PRECON) {
+#pragma unroll
+            for (int k = 0; k < kBlk; k++) M[k] = fm[col0 + k];
         }
         // Per step: loads first and selects instead of branches, so that the
         // loop is straight-line code with no masked stores (the wave 0
