@@ -270,10 +270,8 @@ int init_solver(Solver* s) {
     }
     IFL_HIP(hipMalloc((void**)&s->fmask, (size_t)sk.nstrips * sk.cols * sizeof(uint64_t)));
     IFL_HIP(hipMemsetAsync(s->fmask, 0, (size_t)sk.nstrips * sk.cols * sizeof(uint64_t), s->stream));
-    long ne = 3L * sk.nstrips * W;
-    IFL_HIP(hipMalloc((void**)&s->edge, ne * sizeof(double)));
-    fill_u64_kernel<<<grid_blocks(ne, 256), 256, 0, s->stream>>>((uint64_t*)s->edge, ne, 0x7FF4DEADBEEF1234ull);
-    IFL_LAUNCH_CHECK();
+    IFL_HIP(hipMalloc((void**)&s->edge, 3L * sk.nstrips * W * sizeof(double)));
+    IFL_TRY(edge_reset(s));
     s->npartial = std::max(grid_blocks((long)n, 256), sk.nstrips) + 256;
     IFL_TRY(dalloc(s, &s->partial, s->npartial));
     IFL_HIP(hipMalloc((void**)&s->st, sizeof(PcgState)));
@@ -541,6 +539,16 @@ bool encode_png(const uint8_t* rgba, int w, int h, std::vector<uint8_t>& out) {
 }
 
 }  // namespace
+
+// The sweeps' edge buffers (3 kinds x strips x W) back to the sentinel that
+// marks an edge not yet written; after a sweep that gave up, edges it left
+// behind would otherwise be taken for the next sweep's.
+int ifl::edge_reset(Solver* s) {
+    long ne = 3L * s->sk.nstrips * s->W;
+    fill_u64_kernel<<<grid_blocks(ne, 256), 256, 0, s->stream>>>((uint64_t*)s->edge, ne, 0x7FF4DEADBEEF1234ull);
+    IFL_LAUNCH_CHECK();
+    return IFL_OK;
+}
 
 extern "C" {
 

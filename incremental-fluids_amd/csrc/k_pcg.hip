@@ -325,22 +325,24 @@ IFL_DEV void poller_wave(const StreamArgs& a, const double* ein, double* ering, 
     const int cols = a.sk.cols, W = a.sk.W;
     int t0 = 0;
     long spins = 0;
+    // ring capacity: the edges of finished blocks are no longer read
+    int cap = lds_ld(&hdr->computed) * kBlk + kERing;
     while (t0 < cols) {
-        if (lds_ld(&hdr->abort)) return;
-        // ring capacity: the edges of finished blocks are no longer read
-        const int cap = lds_ld(&hdr->computed) * kBlk + kERing;
         const int t = t0 + lane;
         const int x = edge_x<KIND>(t, cols);
         const bool inwin = t < cols && t < cap;
         const bool need = ein && inwin && x >= 0 && x < W;
         uint64_t v = need ? ld_relaxed(ein + x) : 0;
+        // while the load is in flight: the abort flag, and the capacity for
+        // the next round (read before that round's load: never too large)
+        const int ab = lds_ld(&hdr->abort);
+        const int cap_next = lds_ld(&hdr->computed) * kBlk + kERing;
+        if (ab) return;
         const bool ok = inwin && (!need || v != kSentinel);
         const uint64_t m = __ballot(ok);
         const int n = m == ~0ull ? 64 : (int)__builtin_ctzll(~m);  // ready prefix length
-        if (lane < n) {
-            ering[t % kERing] = as_double(v);
-            if (need) st_relaxed((double*)ein + x, kSentinel);
-        }
+        if (lane < n) ering[t % kERing] = as_double(v);
+        cap = cap_next;
         if (n > 0) {
             t0 += n;
             lds_st(&hdr->edge_ready, t0);
@@ -355,6 +357,13 @@ IFL_DEV void poller_wave(const StreamArgs& a, const double* ein, double* ering, 
                 return;
             }
         }
+    }
+    // Every incoming edge of this sweep has been read: back to the sentinel
+    // for the next sweep of this kind (here rather than inside the poll loop,
+    // where the next poll would wait for each store).
+    if (ein) {
+        double* e = const_cast<double*>(ein);
+        for (int x = lane; x < W; x += 64) e[x] = as_double(kSentinel);
     }
 }
 
@@ -1260,6 +1269,7 @@ int k_project(Solver* s, int limit, ifl_solve_stats* stats) {
     PcgState h = *s->st_host;
     if (h.timeout) {
         s->err = "PCG sweep hand-off timed out";
+        (void)edge_reset(s);
         return IFL_ERR_DEVICE;
     }
     // deferred tails of the last iteration
@@ -1539,6 +1549,7 @@ int k_project_batch(Solver* const* ss, int n, hipStream_t stream, BatchDev** dev
         for (int i = 0; i < n; i++) {
             if (d->states_host[i].timeout) {
                 ss[i]->err = "PCG sweep hand-off timed out";
+                for (int k = 0; k < n; k++) (void)edge_reset(ss[k]);
                 s0->err = ss[i]->err;
                 return IFL_ERR_DEVICE;
             }
