@@ -824,8 +824,18 @@ __global__ __launch_bounds__(256) void matvec_kernel(VecArgs a, int check_done) 
 // Fixed-order sum of `n` partials by one workgroup of 256 threads.
 IFL_DEV double block_sum_partials(const double* partial, int n) {
     __shared__ double red[256];
+    // thread t sums partials t, t + 256, ... in that order; eight loads are
+    // in flight at a time (the adds keep their order)
     double acc = 0.0;
-    for (int k = threadIdx.x; k < n; k += 256) acc += partial[k];
+    int k = threadIdx.x;
+    for (; k + 7 * 256 < n; k += 8 * 256) {
+        double v[8];
+#pragma unroll
+        for (int u = 0; u < 8; u++) v[u] = partial[k + u * 256];
+#pragma unroll
+        for (int u = 0; u < 8; u++) acc += v[u];
+    }
+    for (; k < n; k += 256) acc += partial[k];
     red[threadIdx.x] = acc;
     __syncthreads();
     for (int off = 128; off >= 1; off >>= 1) {
