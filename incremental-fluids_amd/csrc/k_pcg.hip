@@ -851,7 +851,9 @@ IFL_DEV double block_sum_partials(const double* partial, int n) {
 // multiply 64 consecutive cells at a time, and the running sum takes their
 // products in lane order (every lane holds the same sum). Cells the
 // reference skips are skipped, not added as zero.
-IFL_DEV double seq_dot(const Skew& sk, const double* a, const double* b, const uint8_t* flag, int vm) {
+// a_nf, when given, replaces a at non-FLUID cells.
+IFL_DEV double seq_dot(const Skew& sk, const double* a, const double* b, const uint8_t* flag, int vm,
+                       const double* a_nf = nullptr) {
     constexpr int kU = 4;  // cells per lane per round: 256 consecutive cells
     const int lane = threadIdx.x & 63;
     const long n = (long)sk.W * sk.H;
@@ -867,7 +869,7 @@ IFL_DEV double seq_dot(const Skew& sk, const double* a, const double* b, const u
                 const int x = (int)(i % sk.W), y = (int)(i / sk.W);
                 const long p = sk.pos(x, y);
                 in[u] = !vm || flag[p];
-                if (in[u]) pr[u] = a[p] * b[p];
+                if (in[u]) pr[u] = (a_nf && !flag[p] ? a_nf[p] : a[p]) * b[p];
             }
         }
     };
@@ -940,7 +942,9 @@ IFL_DEV void fin_sigma_body(const VecArgs& a, int n, int seq, int init, double t
     double d;
     if (seq) {
         if (threadIdx.x < 64) {
-            const double t = seq_dot(a.sk, a.z, a.r, a.flag, a.vm);
+            // inside the loop the reference's _z at a non-FLUID cell is A s
+            // (levels 4-5 take every cell; this product keeps A s in q)
+            const double t = seq_dot(a.sk, a.z, a.r, a.flag, a.vm, !init && a.pm && !a.vm ? a.q : nullptr);
             if (threadIdx.x == 0) st->dot_tmp = t;
         }
         __syncthreads();
