@@ -101,6 +101,7 @@ struct Solver {
 
     // timing
     bool profiling = false;
+    bool prof_open = false;  // the last prof_begin recorded an event
     Profile prof[IFL_KERNEL_COUNT];
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     struct Timed {
@@ -186,6 +187,7 @@ void batch_dev_free(BatchDev* d);
 // per-launch timing (when profiling): events on the solver's stream around
 // a launch; prof_collect() adds the elapsed times of launches whose index is
 // below the limit given for their kernel, after the stream has completed.
+constexpr int kProfEvery = 8;  // PCG iterations per timed iteration
 int prof_begin(Solver* s, int kernel, int index);
 int prof_end(Solver* s);
 int prof_collect(Solver* s, const int* index_limit);
