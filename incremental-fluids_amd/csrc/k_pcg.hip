@@ -607,8 +607,8 @@ IFL_DEV void writer_wave(const StreamArgs& a, char* ring, RingHdr* hdr, int j, i
             for (int k = 0; k < kBlk; k++) M[k] = fm[col0 + k];
         }
         // Per step: loads first and selects instead of branches, so that the
-        // loop is straight-line code with no masked stores (the wave 0
-        // recurrence on the neighbouring SIMD measurably slows down while
+        // loop is straight-line code with no masked stores (wave 0's
+        // recurrence, on another SIMD of the CU, measurably slows down while
         // this wave runs branchy code or masked stores).
 #pragma unroll
         for (int k = 0; k < kBlk; k++) {
