@@ -69,7 +69,43 @@ def calibrate_copy():
     print(json.dumps({"calibration": "device copy", "bytes_read": n * 8, "bytes_written": n * 8, "launches": 3}))
 
 
+class _OneCore:
+    """Pins the calling thread to one host core (the first it may run on,
+    as `taskset -c <core>` would) while a CPU baseline runs, and describes
+    the host: the CPU model (/proc/cpuinfo, what lscpu reports), its logical
+    CPU count and the core used."""
+
+    def __enter__(self):
+        self.saved = os.sched_getaffinity(0)
+        self.core = min(self.saved)
+        os.sched_setaffinity(0, {self.core})
+        return self
+
+    def __exit__(self, *exc):
+        os.sched_setaffinity(0, self.saved)
+        return False
+
+    def host(self):
+        model = "unknown"
+        try:
+            with open("/proc/cpuinfo") as f:
+                for line in f:
+                    if line.startswith("model name"):
+                        model = line.split(":", 1)[1].strip()
+                        break
+        except OSError:
+            pass
+        return {"cpu_model": model, "nproc": os.cpu_count(), "pinned_core": self.core}
+
+
 def cpu_baseline(scene, n, mean_iters):
+    with _OneCore() as pin:
+        out = _cpu_baseline(scene, n, mean_iters)
+        out["host"] = pin.host()
+    return out
+
+
+def _cpu_baseline(scene, n, mean_iters):
     """Times the reference's solver 7 (compiled from its own sources, or the
     C restatement when that build is absent) on one host core: one update()
     sequence with both PCG budgets capped, plus a run of PCG iterations, and
@@ -200,6 +236,13 @@ def flip_kernel_bytes(n, p_before, p_after):
 
 
 def cpu_baseline_flip(scene, n):
+    with _OneCore() as pin:
+        out = _cpu_baseline_flip(scene, n)
+        out["host"] = pin.host()
+    return out
+
+
+def _cpu_baseline_flip(scene, n):
     """One update() of the reference's solver 8 (or the C restatement) at the
     same size on one host core, from its own constructor."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
@@ -285,6 +328,13 @@ def run_flip(args):
 
 
 def cpu_baseline_batch(insts, sample):
+    with _OneCore() as pin:
+        out = _cpu_baseline_batch(insts, sample)
+        out["host"] = pin.host()
+    return out
+
+
+def _cpu_baseline_batch(insts, sample):
     """One update() of the first `sample` instances on one host core with the
     reference's solver 4 (or the C restatement): cell-updates/s over them."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
