@@ -38,8 +38,12 @@ enum {
 /* ---- cell types, 7:104-107 and 8:115-119 ---- */
 enum { IFL_CELL_FLUID = 0, IFL_CELL_SOLID = 1, IFL_CELL_EMPTY = 2 };
 
-/* ---- solid bodies, SolidBox 7:169-218 / SolidSphere 7:220-257 ---- */
-enum { IFL_BODY_BOX = 0, IFL_BODY_SPHERE = 1 };
+/* ---- solid bodies, SolidBox 7:169-218 / SolidSphere 7:220-257 ----
+ * IFL_BODY_HOST: any other SolidBody subclass. Its three virtuals
+ * (distance, closestSurfacePoint, distanceNormal, 7:143-146) are evaluated
+ * on the host through the callbacks of ifl_set_host_body_ops; its rigid
+ * velocity (velocityX/Y 7:148-154, not virtual) comes from pos/vel below. */
+enum { IFL_BODY_BOX = 0, IFL_BODY_SPHERE = 1, IFL_BODY_HOST = 2 };
 
 typedef struct ifl_body {
     int32_t type;          /* IFL_BODY_BOX or IFL_BODY_SPHERE */
@@ -123,6 +127,26 @@ const char* ifl_version(void);
  * re-reads on every update(). n <= 255 (uint8 _body, 7:268). Level 8 needs
  * the bodies before the first update (initParticles 8:735). */
 int ifl_set_bodies(ifl_solver* s, const ifl_body* bodies, int n);
+
+/* Host evaluation of the bodies' virtuals, for body lists that hold an
+ * IFL_BODY_HOST body (levels 4-7). While such a list is set, every body of it
+ * (built-in ones too) is evaluated through these callbacks, in the
+ * reference's order: fillSolidFields (7:475-522) takes phi at the cell
+ * corners (min over the bodies in list order), the closest body of each
+ * cell (strict <, the lowest index wins ties) and its normal from the host;
+ * advect's backProject (7:424-435) hands the points that land in a solid
+ * cell to `closest` in batches. Each callback gets the body's index in the
+ * list and n points; the library does not keep the pointers. Pass NULL to
+ * clear. Replaces the virtual calls bodies[i]->distance(x, y) (7:483, 493),
+ * ->distanceNormal(nx, ny, x, y) (7:516) and ->closestSurfacePoint(x, y)
+ * (7:431). */
+typedef struct ifl_host_body_ops {
+    void* user;
+    void (*distance)(void* user, int body, const double* x, const double* y, double* out, size_t n);
+    void (*closest)(void* user, int body, double* x, double* y, size_t n);
+    void (*normal)(void* user, int body, const double* x, const double* y, double* nx, double* ny, size_t n);
+} ifl_host_body_ops;
+int ifl_set_host_body_ops(ifl_solver* s, const ifl_host_body_ops* ops);
 
 /* FluidSolver::addInflow(x, y, w, h, d, [t,] u, v), 3:449 / 7:1036.
  * vals = {d, t, u, v}; t is ignored below level 6. */

@@ -14,10 +14,16 @@
  * lines. Solid bodies stay host objects that the caller owns and moves, as
  * in the reference (7:1124-1125); their state is sent with every update().
  *
+ * User-defined SolidBody subclasses derive from SolidBody through the
+ * reference's protected constructor and implement its three virtuals; while
+ * the body list holds one, the solver evaluates every body's virtuals on the
+ * host (ifl_set_host_body_ops) and the device the rest. Levels 4-7; the
+ * device evaluates SolidBox and SolidSphere itself.
+ *
  * Differences: errors throw std::runtime_error (the reference has no error
- * path); user-defined SolidBody subclasses are not supported (the device
- * evaluates boxes and spheres); ~FluidSolver is safe at every level (the
- * reference's level-8 destructor double-deletes, 8:887-888).
+ * path); user-defined bodies are rejected at level 8; ~FluidSolver is safe at
+ * every level (the reference's level-8 destructor double-deletes,
+ * 8:887-888).
  * The environment variable IFL_DOT_ORDER=sequential selects dot products in
  * the reference's summation order.
  */
@@ -45,6 +51,7 @@ inline void check(int rc, const ifl_solver* s) {
     if (rc != IFL_OK) throw std::runtime_error(std::string("ifl: ") + ifl_last_error(s));
 }
 inline double length(double x, double y) { return sqrt(x * x + y * y); }
+inline int nsgn(double a) { return a < 0.0 ? -1 : 1; }
 inline void rotate(double& x, double& y, double phi) {
     double tx = x, ty = y;
     x = cos(phi) * tx + sin(phi) * ty;
@@ -61,6 +68,28 @@ protected:
     double _velX, _velY, _velTheta;
     int _type;
 
+    /* globalToLocal / localToGlobal 7:119-133 */
+    void globalToLocal(double& x, double& y) const {
+        x -= _posX;
+        y -= _posY;
+        ifl_detail::rotate(x, y, -_theta);
+        x /= _scaleX;
+        y /= _scaleY;
+    }
+    void localToGlobal(double& x, double& y) const {
+        x *= _scaleX;
+        y *= _scaleY;
+        ifl_detail::rotate(x, y, _theta);
+        x += _posX;
+        y += _posY;
+    }
+
+    /* the reference's constructor (7:135-138): a user-defined body, whose
+     * virtuals the solver evaluates on the host */
+    SolidBody(double posX, double posY, double scaleX, double scaleY, double theta, double velX, double velY,
+              double velTheta)
+        : _posX(posX), _posY(posY), _scaleX(scaleX), _scaleY(scaleY), _theta(theta), _velX(velX), _velY(velY),
+          _velTheta(velTheta), _type(IFL_BODY_HOST) {}
     SolidBody(double posX, double posY, double scaleX, double scaleY, double theta, double velX, double velY,
               double velTheta, int type)
         : _posX(posX), _posY(posY), _scaleX(scaleX), _scaleY(scaleY), _theta(theta), _velX(velX), _velY(velY),
@@ -68,7 +97,11 @@ protected:
 
 public:
     virtual ~SolidBody() {}
+    /* 7:143-146 */
     virtual double distance(double x, double y) const = 0;
+    virtual void closestSurfacePoint(double& x, double& y) const = 0;
+    virtual void distanceNormal(double& nx, double& ny, double x, double y) const = 0;
+    int deviceType() const { return _type; }
 
     double velocityX(double x, double y) const { (void)x; return (_posY - y) * _velTheta + _velX; }
     double velocityY(double x, double y) const { (void)y; return (x - _posX) * _velTheta + _velY; }
@@ -116,6 +149,35 @@ public:
         if (dx >= 0.0 || dy >= 0.0) return ifl_detail::length(std::max(dx, 0.0), std::max(dy, 0.0));
         return std::max(dx, dy);
     }
+    /* 7:188-203 */
+    void closestSurfacePoint(double& x, double& y) const {
+        x -= _posX;
+        y -= _posY;
+        ifl_detail::rotate(x, y, -_theta);
+        double dx = fabs(x) - _scaleX * 0.5;
+        double dy = fabs(y) - _scaleY * 0.5;
+        if (dx > dy)
+            x = ifl_detail::nsgn(x) * 0.5 * _scaleX;
+        else
+            y = ifl_detail::nsgn(y) * 0.5 * _scaleY;
+        ifl_detail::rotate(x, y, _theta);
+        x += _posX;
+        y += _posY;
+    }
+    /* 7:205-217 */
+    void distanceNormal(double& nx, double& ny, double x, double y) const {
+        x -= _posX;
+        y -= _posY;
+        ifl_detail::rotate(x, y, -_theta);
+        if (fabs(x) - _scaleX * 0.5 > fabs(y) - _scaleY * 0.5) {
+            nx = ifl_detail::nsgn(x);
+            ny = 0.0;
+        } else {
+            nx = 0.0;
+            ny = ifl_detail::nsgn(y);
+        }
+        ifl_detail::rotate(nx, ny, _theta);
+    }
 };
 
 /* SolidSphere 7:220-257 */
@@ -124,6 +186,32 @@ public:
     SolidSphere(double x, double y, double s, double t, double vx, double vy, double vt)
         : SolidBody(x, y, s, s, t, vx, vy, vt, IFL_BODY_SPHERE) {}
     double distance(double x, double y) const { return ifl_detail::length(x - _posX, y - _posY) - _scaleX * 0.5; }
+    /* 7:230-243 */
+    void closestSurfacePoint(double& x, double& y) const {
+        globalToLocal(x, y);
+        double r = ifl_detail::length(x, y);
+        if (r < 1e-4) {
+            x = 0.5;
+            y = 0.0;
+        } else {
+            x /= 2.0 * r;
+            y /= 2.0 * r;
+        }
+        localToGlobal(x, y);
+    }
+    /* 7:245-256 (the radius is rounded to float, as there) */
+    void distanceNormal(double& nx, double& ny, double x, double y) const {
+        x -= _posX;
+        y -= _posY;
+        float r = ifl_detail::length(x, y);
+        if (r < 1e-4) {
+            nx = 1.0;
+            ny = 0.0;
+        } else {
+            nx = x / r;
+            ny = y / r;
+        }
+    }
 };
 
 /* FluidSolver: 3:187-466, 4:590-917, 5:628-970, 6:587-1058, 7:591-1082, 8:942-1467 */
@@ -149,10 +237,38 @@ class FluidSolver {
         /* the FLIP constructor seeds its particles from the bodies (8:1308-1314) */
         if (IFL_SOLVER_LEVEL == 8) ifl_detail::check(ifl_init_particles(_s), _s);
     }
+    /* the virtuals of the caller's bodies, for lists that hold a user-defined body */
+    static const SolidBody* body_at(void* user, int i) {
+        return (*(const std::vector<const SolidBody*>*)user)[i];
+    }
+    static void cb_distance(void* user, int b, const double* x, const double* y, double* out, size_t n) {
+        const SolidBody* body = body_at(user, b);
+        for (size_t i = 0; i < n; i++) out[i] = body->distance(x[i], y[i]);
+    }
+    static void cb_closest(void* user, int b, double* x, double* y, size_t n) {
+        const SolidBody* body = body_at(user, b);
+        for (size_t i = 0; i < n; i++) body->closestSurfacePoint(x[i], y[i]);
+    }
+    static void cb_normal(void* user, int b, const double* x, const double* y, double* nx, double* ny, size_t n) {
+        const SolidBody* body = body_at(user, b);
+        for (size_t i = 0; i < n; i++) body->distanceNormal(nx[i], ny[i], x[i], y[i]);
+    }
     void push_bodies() {
         if (!_bodies) return;
         std::vector<ifl_body> pods;
-        for (size_t i = 0; i < _bodies->size(); i++) pods.push_back((*_bodies)[i]->pod());
+        bool host = false;
+        for (size_t i = 0; i < _bodies->size(); i++) {
+            pods.push_back((*_bodies)[i]->pod());
+            host = host || pods.back().type == IFL_BODY_HOST;
+        }
+        if (host) {
+            ifl_host_body_ops ops;
+            ops.user = (void*)_bodies;
+            ops.distance = cb_distance;
+            ops.closest = cb_closest;
+            ops.normal = cb_normal;
+            ifl_detail::check(ifl_set_host_body_ops(_s, &ops), _s);
+        }
         ifl_detail::check(ifl_set_bodies(_s, pods.empty() ? nullptr : &pods[0], (int)pods.size()), _s);
     }
     static void print_solve(const ifl_solve_stats& st) {

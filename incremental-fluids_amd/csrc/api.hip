@@ -197,7 +197,7 @@ void free_all(Solver* s) {
     F(s->r); F(s->p); F(s->z); F(s->s); F(s->precon); F(s->adiag); F(s->aplusx); F(s->aplusy);
     F(s->cx); F(s->cy); F(s->yv); F(s->qv); F(s->dflag); F(s->fmask); F(s->edge);
     F(s->cx32); F(s->cy32); F(s->pc32); F(s->y32); F(s->partial); F(s->st);
-    F(s->stage_buf); F(s->front); F(s->front_cnt); F(s->bodies_d);
+    F(s->stage_buf); F(s->front); F(s->front_cnt); F(s->bodies_d); F(s->hreq); F(s->hreq_cnt);
     F(s->posx); F(s->posy);
     for (int t = 0; t < 4; t++) F(s->prop[t]);
     F(s->weight); F(s->counts); F(s->image_d); F(s->step_flags);
@@ -293,6 +293,7 @@ int init_solver(Solver* s) {
     IFL_HIP(hipMalloc((void**)&s->front_cnt, 2 * sizeof(int)));
     IFL_HIP(hipMemsetAsync(s->front_cnt, 0, 2 * sizeof(int), s->stream));
     IFL_HIP(hipMalloc((void**)&s->bodies_d, kMaxBodies * sizeof(Body)));
+    IFL_HIP(hipMalloc((void**)&s->hreq_cnt, 4 * sizeof(int)));
     IFL_HIP(hipMalloc((void**)&s->step_flags, 2 * sizeof(int)));
     IFL_HIP(hipHostMalloc((void**)&s->step_flags_host, 2 * sizeof(int), hipHostMallocDefault));
     s->step_flags_host[0] = s->step_flags_host[1] = 0;
@@ -643,6 +644,16 @@ int ifl_set_bodies(ifl_solver* h, const ifl_body* b, int n) {
     if (!h || n < 0 || n > kMaxBodies || (n > 0 && !b)) return IFL_ERR_ARG;
     Solver* s = &h->s;
     if (s->level < 4 && n > 0) return set_err(s, IFL_ERR_UNSUPPORTED, "solver 3 has no solid bodies");
+    bool host = false;
+    for (int i = 0; i < n; i++) {
+        if (b[i].type < IFL_BODY_BOX || b[i].type > IFL_BODY_HOST) return set_err(s, IFL_ERR_ARG, "unknown body type");
+        host = host || b[i].type == IFL_BODY_HOST;
+    }
+    if (host && s->level == 8)
+        return set_err(s, IFL_ERR_UNSUPPORTED, "host-evaluated bodies are not available at level 8");
+    if (host && !(s->hops.distance && s->hops.closest && s->hops.normal))
+        return set_err(s, IFL_ERR_ARG, "a host-evaluated body needs ifl_set_host_body_ops first");
+    s->host_geom = host;
     for (int i = 0; i < n; i++) {
         Body& d = s->bodies_h[i];
         d.type = b[i].type;
@@ -663,6 +674,19 @@ int ifl_set_bodies(ifl_solver* h, const ifl_body* b, int n) {
     IFL_HIP(hipSetDevice(s->device));
     if (n > 0) IFL_HIP(hipMemcpyAsync(s->bodies_d, s->bodies_h, n * sizeof(Body), hipMemcpyHostToDevice, s->stream));
     IFL_HIP(hipStreamSynchronize(s->stream));
+    return IFL_OK;
+}
+
+int ifl_set_host_body_ops(ifl_solver* h, const ifl_host_body_ops* ops) {
+    if (!h) return IFL_ERR_ARG;
+    Solver* s = &h->s;
+    if (ops && !(ops->distance && ops->closest && ops->normal))
+        return set_err(s, IFL_ERR_ARG, "ifl_host_body_ops needs all three callbacks");
+    if (!ops && s->host_geom) return set_err(s, IFL_ERR_ARG, "the body list holds a host-evaluated body");
+    if (ops)
+        s->hops = *ops;
+    else
+        s->hops = ifl_host_body_ops{};
     return IFL_OK;
 }
 

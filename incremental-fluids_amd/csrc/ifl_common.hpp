@@ -14,12 +14,14 @@
 #include "../../include/ifl.h"
 
 #define IFL_DEV __device__ __forceinline__
+// also evaluated on the host (the host-evaluated bodies' solid fields)
+#define IFL_HD __host__ __device__ __forceinline__
 
 namespace ifl {
 
 // std::min / std::max semantics (b < a ? b : a), including NaN behaviour.
-IFL_DEV double smin(double a, double b) { return (b < a) ? b : a; }
-IFL_DEV double smax(double a, double b) { return (a < b) ? b : a; }
+IFL_HD double smin(double a, double b) { return (b < a) ? b : a; }
+IFL_HD double smax(double a, double b) { return (a < b) ? b : a; }
 IFL_DEV int imin(int a, int b) { return (b < a) ? b : a; }
 IFL_DEV int imax(int a, int b) { return (a < b) ? b : a; }
 // sgn / nsgn 7:37-43
@@ -153,13 +155,13 @@ IFL_DEV double body_velx(const Body& b, double y) { return (b.py - y) * b.vt + b
 IFL_DEV double body_vely(const Body& b, double x) { return (x - b.px) * b.vt + b.vy; }
 
 // occupancy 7:60-102
-IFL_DEV double tri_occ(double out1, double in, double out2) {
+IFL_HD double tri_occ(double out1, double in, double out2) {
     return 0.5 * in * in / ((out1 - in) * (out2 - in));
 }
-IFL_DEV double trap_occ(double out1, double out2, double in1, double in2) {
+IFL_HD double trap_occ(double out1, double out2, double in1, double in2) {
     return 0.5 * (-in1 / (out1 - in1) - in2 / (out2 - in2));
 }
-IFL_DEV double occupancy(double d11, double d12, double d21, double d22) {
+IFL_HD double occupancy(double d11, double d12, double d21, double d22) {
     unsigned b = (d11 < 0.0 ? 1u : 0u) | (d12 < 0.0 ? 2u : 0u) | (d22 < 0.0 ? 4u : 0u) | (d21 < 0.0 ? 8u : 0u);
     switch (b) {
     case 0x0: return 0.0;

@@ -79,6 +79,14 @@ struct Solver {
     int nbodies = 0;
     Body bodies_h[kMaxBodies];
     Body* bodies_d = nullptr;
+    // host-evaluated bodies (ifl_set_host_body_ops): while the body list holds
+    // an IFL_BODY_HOST body, fillSolidFields and backProject call back to the
+    // host; `hreq` holds the back-projection requests of one advection
+    ifl_host_body_ops hops{};
+    bool host_geom = false;
+    struct HostReq* hreq = nullptr;
+    int* hreq_cnt = nullptr;  // one counter per quantity
+    size_t hreq_cap = 0;      // requests per quantity
 
     // level 8 particles
     int max_particles = 0;
@@ -144,6 +152,14 @@ struct Solver {
         int rc_ = (call);              \
         if (rc_ != IFL_OK) return rc_; \
     } while (0)
+
+// A point of advect's trace that landed in a solid cell of a host-evaluated
+// body: cell index, body, and the point in world units (7:428-431).
+struct HostReq {
+    int idx;
+    int body;
+    double x, y;
+};
 
 // ---- launchers (k_fields.hip) ----
 int k_fill_solid(Solver* s, int qi);

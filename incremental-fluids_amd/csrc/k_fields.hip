@@ -6,6 +6,8 @@
 // four faces). Here every output element gathers its contributions in the
 // order the reference's row-major loop would have added them, so the sums
 // are bit-identical (accumulation orders cited per kernel).
+#include <algorithm>
+
 #include "ifl_solver.hpp"
 
 namespace ifl {
@@ -59,8 +61,95 @@ __global__ void fill_cells_kernel(QtyView q, const Body* __restrict__ bodies, in
     q.cell[i] = (vol == 0.0) ? IFL_CELL_SOLID : IFL_CELL_FLUID;
 }
 
+// fillSolidFields with host-evaluated bodies (7:475-522): phi, the closest
+// body and its normal come from the host callbacks in the reference's order;
+// volume and cell follow from them exactly as on the device.
+static int fill_solid_host(Solver* s, int qi) {
+    const QtyView q = s->q[qi];
+    const ifl_host_body_ops& ops = s->hops;
+    const int w = q.w, h = q.h, nb = s->nbodies;
+    const long n = (long)w * h, nc = (long)(w + 1) * (h + 1);
+    std::vector<double> xs, ys, tmp, phi;
+    if (s->level >= 5) {
+        xs.resize(nc);
+        ys.resize(nc);
+        for (long i = 0; i < nc; i++) {
+            xs[i] = ((int)(i % (w + 1)) + q.ox - 0.5) * q.hx;
+            ys[i] = ((int)(i / (w + 1)) + q.oy - 0.5) * q.hx;
+        }
+        phi.resize(nc);
+        tmp.resize(nc);
+        ops.distance(ops.user, 0, xs.data(), ys.data(), phi.data(), (size_t)nc);
+        for (int k = 1; k < nb; k++) {
+            ops.distance(ops.user, k, xs.data(), ys.data(), tmp.data(), (size_t)nc);
+            for (long i = 0; i < nc; i++) phi[i] = smin(phi[i], tmp[i]);
+        }
+    }
+    xs.resize(n);
+    ys.resize(n);
+    for (long i = 0; i < n; i++) {
+        xs[i] = ((int)(i % w) + q.ox) * q.hx;
+        ys[i] = ((int)(i / w) + q.oy) * q.hx;
+    }
+    std::vector<double> d(n), nx(n), ny(n), vol;
+    std::vector<uint8_t> body(n, 0), cell(n);
+    tmp.resize(n);
+    ops.distance(ops.user, 0, xs.data(), ys.data(), d.data(), (size_t)n);
+    for (int k = 1; k < nb; k++) {
+        ops.distance(ops.user, k, xs.data(), ys.data(), tmp.data(), (size_t)n);
+        for (long i = 0; i < n; i++)
+            if (tmp[i] < d[i]) {
+                body[i] = (uint8_t)k;
+                d[i] = tmp[i];
+            }
+    }
+    // distanceNormal of each cell's closest body, one batch per body
+    std::vector<long> sel;
+    std::vector<double> bx, by, bnx, bny;
+    for (int k = 0; k < nb; k++) {
+        sel.clear();
+        for (long i = 0; i < n; i++)
+            if (body[i] == k) sel.push_back(i);
+        if (sel.empty()) continue;
+        bx.resize(sel.size());
+        by.resize(sel.size());
+        bnx.resize(sel.size());
+        bny.resize(sel.size());
+        for (size_t j = 0; j < sel.size(); j++) {
+            bx[j] = xs[sel[j]];
+            by[j] = ys[sel[j]];
+        }
+        ops.normal(ops.user, k, bx.data(), by.data(), bnx.data(), bny.data(), sel.size());
+        for (size_t j = 0; j < sel.size(); j++) {
+            nx[sel[j]] = bnx[j];
+            ny[sel[j]] = bny[j];
+        }
+    }
+    if (s->level == 4) {
+        for (long i = 0; i < n; i++) cell[i] = d[i] < 0.0 ? IFL_CELL_SOLID : IFL_CELL_FLUID;  // 4:466-470
+    } else {
+        vol.resize(n);
+        for (long i = 0; i < n; i++) {
+            const long ip = (i % w) + (i / w) * (w + 1);
+            double v = 1.0 - occupancy(phi[ip], phi[ip + 1], phi[ip + w + 1], phi[ip + w + 2]);
+            if (v < 0.01) v = 0.0;
+            vol[i] = v;
+            cell[i] = v == 0.0 ? IFL_CELL_SOLID : IFL_CELL_FLUID;
+        }
+        IFL_HIP(hipMemcpyAsync(q.phi, phi.data(), nc * sizeof(double), hipMemcpyHostToDevice, s->stream));
+        IFL_HIP(hipMemcpyAsync(q.volume, vol.data(), n * sizeof(double), hipMemcpyHostToDevice, s->stream));
+    }
+    IFL_HIP(hipMemcpyAsync(q.body, body.data(), n, hipMemcpyHostToDevice, s->stream));
+    IFL_HIP(hipMemcpyAsync(q.cell, cell.data(), n, hipMemcpyHostToDevice, s->stream));
+    IFL_HIP(hipMemcpyAsync(q.nx, nx.data(), n * sizeof(double), hipMemcpyHostToDevice, s->stream));
+    IFL_HIP(hipMemcpyAsync(q.ny, ny.data(), n * sizeof(double), hipMemcpyHostToDevice, s->stream));
+    IFL_HIP(hipStreamSynchronize(s->stream));  // the host vectors go out of scope
+    return IFL_OK;
+}
+
 int k_fill_solid(Solver* s, int qi) {
     if (s->nbodies == 0) return IFL_OK;  // 7:476-477
+    if (s->host_geom) return fill_solid_host(s, qi);
     QtyView q = s->q[qi];
     if (s->level >= 5) {
         long n = (long)(q.w + 1) * (q.h + 1);
@@ -331,6 +420,10 @@ struct AdvectArgs {
     double hx, dt;
     int level;
     const Body* bodies;
+    // host-evaluated bodies: back-projections are queued here instead
+    HostReq* req;   // hreq_cap requests per quantity
+    int* req_cnt;   // per quantity
+    long req_cap;
 };
 
 // rungeKutta3 7:297-315 + backProject 7:424-435 + cerp 7:404-422, as called
@@ -363,12 +456,77 @@ __global__ void advect_kernel(AdvectArgs a) {
         if (q.cell[ri] != IFL_CELL_FLUID) {
             x = (x - q.ox) * hx;
             y = (y - q.oy) * hx;
+            if (a.req) {
+                // the host evaluates closestSurfacePoint; advect_finish_kernel samples
+                const int k = atomicAdd(&a.req_cnt[blockIdx.y], 1);
+                HostReq* r = a.req + blockIdx.y * a.req_cap + k;
+                r->idx = i;
+                r->body = q.body[ri];
+                r->x = x;
+                r->y = y;
+                return;
+            }
             body_closest(a.bodies[q.body[ri]], x, y);
             x = x / hx + q.ox;
             y = y / hx + q.oy;
         }
     }
     q.dst[i] = qcerp(q.src, q.w, q.h, q.ox, q.oy, x, y);
+}
+
+// the cerp of advect (7:450) at the back-projected points the host returned
+__global__ void advect_finish_kernel(AdvectArgs a) {
+    const QtyView& q = a.q[blockIdx.y];
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= a.req_cnt[blockIdx.y]) return;
+    const HostReq r = a.req[blockIdx.y * a.req_cap + k];
+    const double x = r.x / a.hx + q.ox;
+    const double y = r.y / a.hx + q.oy;
+    q.dst[r.idx] = qcerp(q.src, q.w, q.h, q.ox, q.oy, x, y);
+}
+
+// backProject with host-evaluated bodies: the trace end points that land in a
+// solid cell go to the host's closestSurfacePoint, one batch per body
+static int advect_host_finish(Solver* s, AdvectArgs& a, long maxn) {
+    int cnt[4] = {0, 0, 0, 0};
+    IFL_HIP(hipMemcpyAsync(cnt, a.req_cnt, a.nq * sizeof(int), hipMemcpyDeviceToHost, s->stream));
+    IFL_HIP(hipStreamSynchronize(s->stream));
+    int most = 0;
+    std::vector<HostReq> req;
+    std::vector<double> xs, ys;
+    std::vector<long> sel;
+    for (int qi = 0; qi < a.nq; qi++) {
+        if (cnt[qi] == 0) continue;
+        most = std::max(most, cnt[qi]);
+        req.resize(cnt[qi]);
+        HostReq* dev = a.req + qi * a.req_cap;
+        IFL_HIP(hipMemcpy(req.data(), dev, cnt[qi] * sizeof(HostReq), hipMemcpyDeviceToHost));
+        for (int b = 0; b < s->nbodies; b++) {
+            sel.clear();
+            for (int k = 0; k < cnt[qi]; k++)
+                if (req[k].body == b) sel.push_back(k);
+            if (sel.empty()) continue;
+            xs.resize(sel.size());
+            ys.resize(sel.size());
+            for (size_t j = 0; j < sel.size(); j++) {
+                xs[j] = req[sel[j]].x;
+                ys[j] = req[sel[j]].y;
+            }
+            s->hops.closest(s->hops.user, b, xs.data(), ys.data(), sel.size());
+            for (size_t j = 0; j < sel.size(); j++) {
+                req[sel[j]].x = xs[j];
+                req[sel[j]].y = ys[j];
+            }
+        }
+        IFL_HIP(hipMemcpy(dev, req.data(), cnt[qi] * sizeof(HostReq), hipMemcpyHostToDevice));
+    }
+    (void)maxn;
+    if (most > 0) {
+        dim3 grid(grid_blocks(most, 256), a.nq);
+        advect_finish_kernel<<<grid, 256, 0, s->stream>>>(a);
+        IFL_LAUNCH_CHECK();
+    }
+    return IFL_OK;
 }
 
 int k_advect(Solver* s, int qmask, double dt) {
@@ -390,10 +548,29 @@ int k_advect(Solver* s, int qmask, double dt) {
     a.dt = dt;
     a.level = s->level;
     a.bodies = s->bodies_d;
+    a.req = nullptr;
+    a.req_cnt = nullptr;
+    a.req_cap = 0;
+    const bool host = s->host_geom && s->level >= 4 && s->nbodies > 0;
+    if (host) {
+        // every FLUID cell of a quantity may need a back-projection
+        if ((long)s->hreq_cap < maxn) {
+            if (s->hreq) IFL_HIP(hipFree(s->hreq));
+            s->hreq = nullptr;
+            s->hreq_cap = 0;
+            IFL_HIP(hipMalloc((void**)&s->hreq, 4 * maxn * sizeof(HostReq)));
+            s->hreq_cap = maxn;
+        }
+        a.req = s->hreq;
+        a.req_cnt = s->hreq_cnt;
+        a.req_cap = (long)s->hreq_cap;
+        IFL_HIP(hipMemsetAsync(s->hreq_cnt, 0, 4 * sizeof(int), s->stream));
+    }
     dim3 grid(grid_blocks(maxn, 256), a.nq);
     IFL_TRY(prof_begin(s, IFL_KERNEL_ADVECT, -1));
     advect_kernel<<<grid, 256, 0, s->stream>>>(a);
     IFL_LAUNCH_CHECK();
+    if (host) IFL_TRY(advect_host_finish(s, a, maxn));
     return prof_end(s);
 }
 
