@@ -209,6 +209,16 @@ int ifl_batch_destroy(ifl_batch* batch);
  * render_heat at levels 6-8 (the heat panel left of the density panel). */
 int ifl_to_image(ifl_solver* s, uint8_t* rgba, size_t bytes, int render_heat);
 
+/* No counterpart in the reference: ifl_to_image for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc). It renders the same bytes as
+ * ifl_to_image, staged in row windows through a 4 MiB device scratch array
+ * that every device the library runs on holds for the life of the process;
+ * it calls no hipMalloc or hipFree. One image row (w*4 bytes, w*8 with heat)
+ * must fit in the scratch. All calls on one device share that scratch:
+ * calls that overlap on one device (solvers on other threads) race. */
+int ifl_to_image_static(ifl_solver* s, uint8_t* rgba, size_t bytes, int render_heat);
+
 /* The reference's frame file (lodepng_encode32_file, 7:1122): an RGBA, 8-bit
  * PNG of w x h pixels (w doubled for a heat frame), stored deflate blocks. */
 int ifl_write_png(const char* path, const uint8_t* rgba, int w, int h);

@@ -26,6 +26,36 @@ struct ifl_batch {
 
 namespace ifl {
 
+// Device scratch of the allocation-free entry point ifl_to_image_static: the
+// rendered rows are staged here. Each device holds its own copy for the life
+// of the process. In host code `scratch` names a host-side stand-in, never
+// the device memory: every use goes through g_scratch_dev[device], set once
+// per device by scratch_init (hipGetSymbolAddress with that device current).
+constexpr size_t kScratchBytes = (size_t)4 << 20;
+static __device__ __align__(256) uint8_t scratch[kScratchBytes];
+constexpr int kMaxDevices = 64;
+static std::mutex g_scratch_mu;
+static void* g_scratch_dev[kMaxDevices];
+
+// With `device` current. Idempotent; the address is kept per device.
+static int scratch_init(Solver* s) {
+    if (s->device < 0 || s->device >= kMaxDevices) {
+        s->err = "device ordinal beyond the scratch table";
+        return IFL_ERR_ARG;
+    }
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (g_scratch_dev[s->device]) return IFL_OK;
+    void* ptr = nullptr;
+    IFL_HIP(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+    g_scratch_dev[s->device] = ptr;
+    return IFL_OK;
+}
+
+static void* scratch_ptr(int device) {
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    return (device >= 0 && device < kMaxDevices) ? g_scratch_dev[device] : nullptr;
+}
+
 // frand() 8:38-46 keeps one static seed for the whole process; every level-8
 // solver draws from it in creation and step order, as in the reference.
 static std::mutex g_frand_mu;
@@ -218,6 +248,7 @@ void free_all(Solver* s) {
 
 int init_solver(Solver* s) {
     IFL_HIP(hipSetDevice(s->device));
+    IFL_TRY(scratch_init(s));
     IFL_HIP(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking));
     IFL_HIP(hipEventCreate(&s->ev0));
     IFL_HIP(hipEventCreate(&s->ev1));
@@ -804,9 +835,33 @@ int ifl_to_image(ifl_solver* h, uint8_t* rgba, size_t bytes, int render_heat) {
         IFL_HIP(hipMalloc((void**)&s->image_d, need));
         s->image_bytes = need;
     }
-    IFL_TRY(k_to_image(s, s->image_d, heat));
+    IFL_TRY(k_to_image(s, s->image_d, heat, 0, s->H));
     IFL_HIP(hipMemcpyAsync(rgba, s->image_d, need, hipMemcpyDeviceToHost, s->stream));
     IFL_HIP(hipStreamSynchronize(s->stream));
+    return IFL_OK;
+}
+
+int ifl_to_image_static(ifl_solver* h, uint8_t* rgba, size_t bytes, int render_heat) {
+    if (!h || !rgba) return IFL_ERR_ARG;
+    Solver* s = &h->s;
+    const int heat = render_heat && s->level >= 6;
+    const size_t row = (size_t)s->W * 4 * (heat ? 2 : 1);
+    if (bytes != row * s->H)
+        return set_err(s, IFL_ERR_ARG, "ifl_to_image_static: buffer must hold w*h*4 bytes (w*h*8 with heat)");
+    if (row > kScratchBytes)
+        return set_err(s, IFL_ERR_ARG, "ifl_to_image_static: one image row exceeds the 4 MiB device scratch "
+                                       "(w at most 1048576, 524288 with heat)");
+    IFL_HIP(hipSetDevice(s->device));
+    uint8_t* ptr = (uint8_t*)scratch_ptr(s->device);
+    if (!ptr) return set_err(s, IFL_ERR_HIP, "ifl_to_image_static: device scratch not set up");
+    // every byte of a window is written by the kernel: nothing to clear
+    const int rows = (int)std::min<size_t>(kScratchBytes / row, (size_t)s->H);
+    for (int y0 = 0; y0 < s->H; y0 += rows) {
+        const int n = std::min(rows, s->H - y0);
+        IFL_TRY(k_to_image(s, ptr, heat, y0, n));
+        IFL_HIP(hipMemcpyAsync(rgba + (size_t)y0 * row, ptr, (size_t)n * row, hipMemcpyDeviceToHost, s->stream));
+        IFL_HIP(hipStreamSynchronize(s->stream));  // the next window reuses the scratch
+    }
     return IFL_OK;
 }
 

@@ -178,7 +178,7 @@ int k_copy_p_to_t(Solver* s);
 int k_update_dflag(Solver* s);
 int k_max_timestep(Solver* s, double* dt_out);  // solver 1's CFL bound
 int k_nonfinite(Solver* s, int* dev_flag);      // any NaN / infinity in d, t, u, v
-int k_to_image(Solver* s, uint8_t* dev_rgba, int heat);  // toImage into device memory
+int k_to_image(Solver* s, uint8_t* dev_rgba, int heat, int y0, int rows);  // toImage rows y0.. into device memory
 int edge_reset(Solver* s);  // the sweeps' edge buffers back to "not yet written" (api.hip)
 // skewed <-> row-major conversion through s->stage_buf
 int k_skew_to_rows(Solver* s, const double* skew, double* rows);

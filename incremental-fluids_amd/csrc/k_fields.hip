@@ -346,28 +346,32 @@ struct ImageArgs {
     const double* t;
     double t_amb;
     int W, H, level, heat;
-    uint8_t* rgba;
+    uint8_t* rgba;  // the image rows from y0 on
+    int y0, rows;   // the rows this launch renders
 };
 
 __global__ void to_image_kernel(ImageArgs a) {
-    long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= (long)a.W * a.H) return;
+    const long k = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= (long)a.W * a.rows) return;
+    const long i = (long)a.y0 * a.W + k;
     const int x = (int)(i % a.W), y = (int)(i / a.W), lv = a.level;
+    // a.rgba holds the image from row y0 on
     uint8_t* rgba = a.rgba;
+    const int yl = y - a.y0;
     if (lv <= 5) {
         int shade = lv == 5 ? x86_int((1.0 - a.d[i]) * a.volume[i] * 255.0) : x86_int((1.0 - a.d[i]) * 255.0);
         shade = imax(imin(shade, 255), 0);
         if (lv == 4 && a.cell[i] == IFL_CELL_SOLID) shade = 0;  // 4:908-909
-        rgba[i * 4 + 0] = rgba[i * 4 + 1] = rgba[i * 4 + 2] = (uint8_t)shade;
-        rgba[i * 4 + 3] = 0xFF;
+        rgba[k * 4 + 0] = rgba[k * 4 + 1] = rgba[k * 4 + 2] = (uint8_t)shade;
+        rgba[k * 4 + 3] = 0xFF;
         return;
     }
     long idxl = 0, idxr;
     if (a.heat) {
-        idxl = 4 * ((long)x + (long)y * a.W * 2);
-        idxr = 4 * ((long)x + (long)y * a.W * 2 + a.W);
+        idxl = 4 * ((long)x + (long)yl * a.W * 2);
+        idxr = 4 * ((long)x + (long)yl * a.W * 2 + a.W);
     } else {
-        idxr = 4 * i;
+        idxr = 4 * k;
     }
     const double volume = a.volume[i];
     double shade = (1.0 - a.d[i]) * volume;
@@ -392,7 +396,7 @@ __global__ void to_image_kernel(ImageArgs a) {
     }
 }
 
-int k_to_image(Solver* s, uint8_t* dev_rgba, int heat) {
+int k_to_image(Solver* s, uint8_t* dev_rgba, int heat, int y0, int rows) {
     ImageArgs a;
     a.d = s->q[IFL_Q_D].src;
     a.volume = s->q[IFL_Q_D].volume;
@@ -404,7 +408,9 @@ int k_to_image(Solver* s, uint8_t* dev_rgba, int heat) {
     a.level = s->level;
     a.heat = heat && s->level >= 6;
     a.rgba = dev_rgba;
-    long n = (long)s->W * s->H;
+    a.y0 = y0;
+    a.rows = rows;
+    long n = (long)s->W * rows;
     to_image_kernel<<<grid_blocks(n, 256), 256, 0, s->stream>>>(a);
     IFL_LAUNCH_CHECK();
     return IFL_OK;
