@@ -85,7 +85,6 @@ static int sweep_blocks(int n) { return 8 * ((n + 7) / 8); }
 // the factor sweep stores cx = cy = +0 there, so a neighbour the reference
 // skips contributes exactly cx*(+0) = +0, the same as a skipped term.
 constexpr int kBlk = 8;
-constexpr int kHalf = 4;
 constexpr int kLoaders = 2;
 constexpr int kWaves = 6;
 enum { kWaveCompute = 0, kWaveWriter = 1, kWaveLoader0 = 2, kWaveIdle = 4, kWavePoller = 5 };
@@ -367,19 +366,24 @@ IFL_DEV void poller_wave(const StreamArgs& a, const double* ein, double* ering, 
     }
 }
 
-// Wave 0. Operand registers are double-buffered by half block (4 steps):
-// a half first issues the LDS reads of the next half, then runs its steps.
-template <int KIND, int MODE, int NG, typename T>
+// Wave 0. Operand registers are double-buffered by block (8 steps): a block
+// first issues the LDS reads of the next block, then runs its steps (against
+// half-block buffers: 382 / 371 us instead of 408 / 416 us per forward /
+// backward sweep at 2048^2, with the trace stamps compiled out).
+// TR: the diagnostic build with per-block time stamps (ifl_debug_trace_sweep);
+// the stamps' branches cost ~6 % of a sweep, so the solver's own sweeps are
+// built without them.
+template <int KIND, int MODE, int NG, typename T, bool TR>
 IFL_DEV void compute_wave(const StreamArgs& a, char* ring, RingHdr* hdr, const double* ering, int j, int lane) {
     typedef SweepShape<KIND, MODE, NG, T> S;
     typedef typename Vec2<T>::type V2;
     constexpr int NT = S::NT;
-    const int cols = a.sk.cols, nblk = cols / kBlk, nhalf = 2 * nblk;
+    const int cols = a.sk.cols, nblk = cols / kBlk;
     const const_u64* fm = strip_masks(a, j);
     const double tau = a.tau, sigma = a.sigma;
-    long long* tr = a.trace ? a.trace + (long)j * nblk * kTraceStamps : nullptr;
+    long long* tr = TR ? a.trace + (long)j * nblk * kTraceStamps : nullptr;
     auto stamp = [&](int b, int i) {
-        if (tr && lane == 0) tr[b * kTraceStamps + i] = __builtin_amdgcn_s_memrealtime();
+        if (TR && lane == 0) tr[b * kTraceStamps + i] = __builtin_amdgcn_s_memrealtime();
     };
 
     // Readiness is cached: the counters are re-read only when the cached
@@ -411,25 +415,27 @@ IFL_DEV void compute_wave(const StreamArgs& a, char* ring, RingHdr* hdr, const d
         }
     };
 
-    // Operands of a half, shaped as the loads deliver them so that the
+    // Operands of a block, shaped as the loads deliver them so that the
     // loop-carried registers are the load destinations (no copies, which
     // would wait for the loads): X[i][p] holds steps 2p and 2p+1 of the i-th
     // T array (one ds_read2st64), Q[p] those of the updated residual (forward
-    // sweep), E[p] the two edge values of those steps, M the four lane masks
-    // in sweep-column order (one scalar load).
+    // sweep), E[p] the two edge values of those steps, M0/M1 the eight lane
+    // masks in column order (scalar loads). The reads of block b+1 go out at
+    // the start of block b.
     typedef uint64_t vec4u __attribute__((ext_vector_type(4)));
     typedef const __attribute__((address_space(4))) vec4u const_u64x4;
-    V2 XA[NT][2], XB[NT][2];
-    vec2d QA[2], QB[2], EA[2], EB[2];
-    vec4u MA, MB;
-    auto load_half = [&](const char* slot, int h, V2 (&X)[NT][2], vec2d (&Q)[2], vec2d (&E)[2], vec4u& M) {
-        // past the last half (the unused prefetch) stay on the strip's own
-        // masks and edge ring
-        const int hm = h < nhalf ? h : nhalf - 1;
+    constexpr int NP = kBlk / 2;
+    V2 XA[NT][NP], XB[NT][NP];
+    vec2d QA[NP], QB[NP], EA[NP], EB[NP];
+    vec4u MA0, MA1, MB0, MB1;
+    auto load_blk = [&](const char* slot, int b, V2 (&X)[NT][NP], vec2d (&Q)[NP], vec2d (&E)[NP], vec4u& M0, vec4u& M1) {
+        const int bm = b < nblk ? b : nblk - 1;
+        // a block's 8 edges are contiguous in the ring (8 divides its size)
+        const double* eb = ering + ((unsigned)(bm * kBlk) & (unsigned)(kERing - 1));
 #pragma unroll
-        for (int p = 0; p < 2; p++) {
-            const int e0 = step_in_block<KIND>((h & 1) * kHalf + 2 * p) * kStrip + lane;
-            const int e1 = step_in_block<KIND>((h & 1) * kHalf + 2 * p + 1) * kStrip + lane;
+        for (int p = 0; p < NP; p++) {
+            const int e0 = step_in_block<KIND>(2 * p) * kStrip + lane;
+            const int e1 = step_in_block<KIND>(2 * p + 1) * kStrip + lane;
 #pragma unroll
             for (int i = 0; i < NT; i++) {
                 const T* arr = (const T*)(slot + S::ro(S::tarr(i)));
@@ -441,41 +447,35 @@ IFL_DEV void compute_wave(const StreamArgs& a, char* ring, RingHdr* hdr, const d
                 Q[p].x = arr[e0];
                 Q[p].y = arr[e1];
             }
-            E[p] = *(const vec2d*)(ering + (hm * kHalf + 2 * p) % kERing);
+            E[p] = *(const vec2d*)(eb + 2 * p);
         }
-        // the 4 columns of the half, lowest first
-        const int c0 = KIND == SW_BWD ? step_col<KIND>(hm * kHalf + kHalf - 1, cols) : step_col<KIND>(hm * kHalf, cols);
-        M = *(const const_u64x4*)(fm + c0);
+        const int c0 = block_col0<KIND>(bm, cols);  // the block's 8 columns, lowest first
+        M0 = *(const const_u64x4*)(fm + c0);
+        M1 = *(const const_u64x4*)(fm + c0 + 4);
     };
-    auto mask_of = [&](const vec4u& M, int k) -> uint64_t {
-        const int i = KIND == SW_BWD ? kHalf - 1 - k : k;
-        return i == 0 ? M.x : i == 1 ? M.y : i == 2 ? M.z : M.w;
+    auto mask_of = [&](const vec4u& M0, const vec4u& M1, int k) -> uint64_t {
+        const int i = KIND == SW_BWD ? kBlk - 1 - k : k;
+        const vec4u& M = i < 4 ? M0 : M1;
+        const int c = i & 3;
+        return c == 0 ? M.x : c == 1 ? M.y : c == 2 ? M.z : M.w;
     };
     T Pw = 0, Pn = 0, zs = 0;
-    // half h from (X, Q, E, M); first the operands of half h+1 (in nslot) into (Y, P, F, N)
-    auto run_half = [&](const char* slot, int h, const V2 (&X)[NT][2], const vec2d (&Q)[2], const vec2d (&E)[2],
-                        const vec4u& M, const char* nslot, V2 (&Y)[NT][2], vec2d (&P)[2], vec2d (&F)[2], vec4u& N) {
+    auto run_blk = [&](const char* slot, int b, const V2 (&X)[NT][NP], const vec2d (&Q)[NP], const vec2d (&E)[NP],
+                       const vec4u& M0, const vec4u& M1, const char* nslot, V2 (&Y)[NT][NP], vec2d (&P)[NP],
+                       vec2d (&F)[NP], vec4u& N0, vec4u& N1) {
         T* out = (T*)(slot + S::OUT);
-        // Everything this half uses arrived during the previous half. Drain
-        // LDS and scalar loads now, before the next half's reads go out:
-        // scalar loads return out of order, so a wait for the masks placed
-        // after those reads would wait for all of them.
         __builtin_amdgcn_s_waitcnt(0xC07F);  // lgkmcnt(0)
-        // unconditional (past the last block it reads a valid, unused slot):
-        // a conditional load would merge loaded and unloaded values and cost
-        // register copies that wait for the loads
-        load_half(nslot, h + 1, Y, P, F, N);
+        load_blk(nslot, b + 1, Y, P, F, N0, N1);
         __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
-        for (int k = 0; k < kHalf; k++) {
-            const int e = step_in_block<KIND>((h & 1) * kHalf + k) * kStrip + lane;
-            const bool f = lane_on(mask_of(M, k));
+        for (int k = 0; k < kBlk; k++) {
+            const int e = step_in_block<KIND>(k) * kStrip + lane;
+            const bool f = lane_on(mask_of(M0, M1, k));
             const int p = k >> 1;
             const bool hi = k & 1;
             auto x = [&](int i) -> T { return hi ? X[i][p].y : X[i][p].x; };
             const T ek = (T)(hi ? E[p].y : E[p].x);
             if (KIND == SW_FWD) {
-                // applyPreconditioner forward loop 7:773-787 on the updated residual
                 const T Ps = lane_shr1(Pn, ek);
                 const T t1 = (T)(hi ? Q[p].y : Q[p].x) - Pw;
                 const T tt = t1 - Ps;
@@ -485,7 +485,6 @@ IFL_DEV void compute_wave(const StreamArgs& a, char* ring, RingHdr* hdr, const d
                 Pn = x(1) * zm;
                 out[e] = zm;
             } else if (KIND == SW_BWD) {
-                // applyPreconditioner backward loop 7:789-803
                 const T zn = lane_shl1(zs, ek);
                 const T t1 = x(0) - x(1) * zs;
                 const T tt = t1 - x(2) * zn;
@@ -493,7 +492,6 @@ IFL_DEV void compute_wave(const StreamArgs& a, char* ring, RingHdr* hdr, const d
                 zs = f ? zz : (T)0;
                 out[e] = zs;
             } else {
-                // buildPreconditioner 7:744-767 (T is double)
                 const double Ns = lane_shr1((double)Pn, (double)ek);
                 const double ad = x(0);
                 double ev = ad;
@@ -511,41 +509,43 @@ IFL_DEV void compute_wave(const StreamArgs& a, char* ring, RingHdr* hdr, const d
             }
         }
     };
-
-    // the strip's first masks, fetched while it waits for its first edges
-    // (otherwise the first half would wait for them from HBM)
     {
-        const int c0 = KIND == SW_BWD ? step_col<KIND>(kHalf - 1, cols) : step_col<KIND>(0, cols);
+        const int c0 = block_col0<KIND>(0, cols);
         const vec4u m = *(const const_u64x4*)(fm + c0);
         asm volatile("" : : "s"(m.x));
     }
-    if (!ready(0, kHalf)) return;
+    if (!ready(0, kBlk)) return;
     const char* slot = ring;  // block 0 is in slot 0
-    load_half(slot, 0, XA, QA, EA, MA);
+    load_blk(slot, 0, XA, QA, EA, MA0, MA1);
     int sidx = 0;
-    for (int b = 0; b < nblk; b++) {
+    auto next_slot = [&]() {
+        sidx = sidx + 1 == a.nb ? 0 : sidx + 1;
+        return ring + (long)sidx * a.slot_bytes;
+    };
+    for (int b = 0; b < nblk; b += 2) {
+        // block b from A; block b+1 goes to B
         stamp(b, 0);
-        // second half of this block: same slot
-        if (!ready(b, b * kBlk + kBlk)) return;
+        if (b + 1 < nblk && !ready(b + 1, (b + 2) * kBlk)) return;
         stamp(b, 1);
-        run_half(slot, 2 * b, XA, QA, EA, MA, slot, XB, QB, EB, MB);
-        stamp(b, 2);
-        // first half of the next block
-        const int nidx = sidx + 1 == a.nb ? 0 : sidx + 1;
-        const char* nslot = ring + (long)nidx * a.slot_bytes;
-        const bool more = b + 1 < nblk;
-        if (more && !ready(b + 1, b * kBlk + kBlk + kHalf)) return;
-        stamp(b, 3);
-        run_half(slot, 2 * b + 1, XB, QB, EB, MB, nslot, XA, QA, EA, MA);
+        const char* nslot = next_slot();
+        run_blk(slot, b, XA, QA, EA, MA0, MA1, nslot, XB, QB, EB, MB0, MB1);
         lds_st(&hdr->computed, b + 1);
         stamp(b, 4);
         slot = nslot;
-        sidx = nidx;
+        if (b + 1 >= nblk) break;
+        stamp(b + 1, 0);
+        if (b + 2 < nblk && !ready(b + 2, (b + 3) * kBlk)) return;
+        stamp(b + 1, 1);
+        nslot = next_slot();
+        run_blk(slot, b + 1, XB, QB, EB, MB0, MB1, nslot, XA, QA, EA, MA0, MA1);
+        lds_st(&hdr->computed, b + 2);
+        stamp(b + 1, 4);
+        slot = nslot;
     }
 }
 
 // The writer wave: drains finished blocks.
-template <int KIND, int MODE, int LV, int NG, typename T>
+template <int KIND, int MODE, int LV, int NG, typename T, bool TR>
 IFL_DEV void writer_wave(const StreamArgs& a, char* ring, RingHdr* hdr, int j, int lane, double* eout) {
     typedef SweepShape<KIND, MODE, NG, T> S;
     constexpr bool VM = LV >= 6;   // vector operations masked to FLUID cells
@@ -674,7 +674,7 @@ IFL_DEV void writer_wave(const StreamArgs& a, char* ring, RingHdr* hdr, int j, i
             store_block(a.out[1], in(1), 8);
             store_block(a.out[2], in(2), 8);
         }
-        if (a.trace && lane == 0) a.trace[((long)j * nblk + b) * kTraceStamps + 5] = __builtin_amdgcn_s_memrealtime();
+        if (TR && lane == 0) a.trace[((long)j * nblk + b) * kTraceStamps + 5] = __builtin_amdgcn_s_memrealtime();
         lds_st(&hdr->consumed, b + 1);
     }
     if (KIND == SW_FWD) {
@@ -686,7 +686,7 @@ IFL_DEV void writer_wave(const StreamArgs& a, char* ring, RingHdr* hdr, int j, i
     }
 }
 
-template <int KIND, int MODE, int LV, int NG, typename T>
+template <int KIND, int MODE, int LV, int NG, typename T, bool TR>
 IFL_DEV void sweep_body(const StreamArgs& a, int j) {
     extern __shared__ __align__(16) char smem[];
     PcgState* st = a.st;
@@ -707,20 +707,20 @@ IFL_DEV void sweep_body(const StreamArgs& a, int j) {
     double* eout = has_out ? a.edge + (long)(KIND == SW_BWD ? j : j + 1) * W : nullptr;
 
     if (wave == kWaveCompute)
-        compute_wave<KIND, MODE, NG, T>(a, ring, hdr, ering, j, lane);
+        compute_wave<KIND, MODE, NG, T, TR>(a, ring, hdr, ering, j, lane);
     else if (wave == kWaveWriter)
-        writer_wave<KIND, MODE, LV, NG, T>(a, ring, hdr, j, lane, eout);
+        writer_wave<KIND, MODE, LV, NG, T, TR>(a, ring, hdr, j, lane, eout);
     else if (wave == kWavePoller)
         poller_wave<KIND>(a, ein, ering, hdr, lane);
     else if (wave >= kWaveLoader0 && wave < kWaveLoader0 + kLoaders)
         loader_wave<KIND, MODE, LV, NG, T>(a, ring, hdr, wave - kWaveLoader0, lane, sbase, nblk, j);
 }
 
-template <int KIND, int MODE, int LV, int NG, typename T>
+template <int KIND, int MODE, int LV, int NG, typename T, bool TR>
 __global__ __launch_bounds__(64 * kWaves) void stream_sweep_kernel(StreamArgs a) {
     const int j = strip_of_block(blockIdx.x, a.strip0, a.nlaunch);
     if (j < 0) return;
-    sweep_body<KIND, MODE, LV, NG, T>(a, j);
+    sweep_body<KIND, MODE, LV, NG, T, TR>(a, j);
 }
 
 // ---------------------------------------------------------------- vector phases
@@ -847,55 +847,57 @@ IFL_DEV double block_sum_partials(const double* partial, int n) {
 
 // dotProduct in the reference's left-to-right order (dot_order = SEQUENTIAL).
 // The reference's left-to-right sum (7:806-814, masked from level 6 on) in
-// row-major cell order, evaluated by one whole wave: the lanes load and
-// multiply 64 consecutive cells at a time, and the running sum takes their
-// products in lane order (every lane holds the same sum). Cells the
-// reference skips are skipped, not added as zero.
-// a_nf, when given, replaces a at non-FLUID cells.
+// row-major cell order, by one workgroup of 256 threads: waves 1-3 form the
+// products of the next chunk of cells into LDS while wave 0 adds the current
+// chunk in order, two products per LDS read (every lane of wave 0 holds the
+// same running sum). A cell the reference skips contributes +0.0, which
+// leaves the sum unchanged bit for bit: the sum starts at +0.0 and a
+// round-to-nearest sum is never -0.0 unless both addends are, so x + (+0.0)
+// is x for every running sum x. a_nf, when given, replaces a at non-FLUID
+// cells. Every thread of the block must call it; the result is valid in
+// wave 0.
+constexpr int kSeqPer = 8;                  // products per producer thread per chunk
+constexpr int kSeqChunk = 192 * kSeqPer;    // cells per chunk
 IFL_DEV double seq_dot(const Skew& sk, const double* a, const double* b, const uint8_t* flag, int vm,
                        const double* a_nf = nullptr) {
-    constexpr int kU = 4;  // cells per lane per round: 256 consecutive cells
-    const int lane = threadIdx.x & 63;
+    __shared__ __align__(16) double buf[2][kSeqChunk];
     const long n = (long)sk.W * sk.H;
-    double prod[2][kU];
-    bool inc[2][kU];
-    auto load = [&](long base, double (&pr)[kU], bool (&in)[kU]) {
+    const int tid = threadIdx.x;
+    const bool producer = tid >= 64;
+    auto produce = [&](long base, double* dst) {
+        const int t = tid - 64;
 #pragma unroll
-        for (int u = 0; u < kU; u++) {
-            const long i = base + u * 64 + lane;
-            pr[u] = 0.0;
-            in[u] = false;
+        for (int k = 0; k < kSeqPer; k++) {
+            const int j = t + 192 * k;
+            const long i = base + j;
+            double pr = 0.0;
             if (i < n) {
                 const int x = (int)(i % sk.W), y = (int)(i / sk.W);
                 const long p = sk.pos(x, y);
-                in[u] = !vm || flag[p];
-                if (in[u]) pr[u] = (a_nf && !flag[p] ? a_nf[p] : a[p]) * b[p];
+                if (!vm || flag[p]) pr = (a_nf && !flag[p] ? a_nf[p] : a[p]) * b[p];
             }
+            dst[j] = pr;
         }
     };
     double result = 0.0;
-    auto add = [&](const double (&pr)[kU], const bool (&in)[kU]) {
-#pragma unroll
-        for (int u = 0; u < kU; u++) {
-            const int lo = __double2loint(pr[u]), hi = __double2hiint(pr[u]);
-            uint64_t m = __ballot(in[u]);
-            while (m) {
-                const int k = __builtin_ctzll(m);
-                m &= m - 1;
-                result += __hiloint2double(__builtin_amdgcn_readlane(hi, k), __builtin_amdgcn_readlane(lo, k));
+    const long nchunks = (n + kSeqChunk - 1) / kSeqChunk;
+    if (producer) produce(0, buf[0]);
+    __syncthreads();
+    for (long c = 0; c < nchunks; c++) {
+        if (producer) {
+            if (c + 1 < nchunks) produce((c + 1) * kSeqChunk, buf[(c + 1) & 1]);
+        } else {
+            const vec2d* src = (const vec2d*)buf[c & 1];
+            // cells past the end were formed as +0.0: summing the whole chunk
+            // changes nothing
+#pragma unroll 8
+            for (int k = 0; k < kSeqChunk / 2; k++) {
+                const vec2d v = src[k];
+                result += v.x;
+                result += v.y;
             }
         }
-    };
-    // two register sets: a round's loads go out before the previous round's
-    // additions
-    const long step = 64 * kU;
-    load(0, prod[0], inc[0]);
-    for (long base = 0; base < n; base += 2 * step) {
-        if (base + step < n) load(base + step, prod[1], inc[1]);
-        add(prod[0], inc[0]);
-        if (base + step >= n) break;
-        if (base + 2 * step < n) load(base + 2 * step, prod[0], inc[0]);
-        add(prod[1], inc[1]);
+        __syncthreads();
     }
     return result;
 }
@@ -906,10 +908,8 @@ IFL_DEV void fin_alpha_body(const VecArgs& a, int n, int seq) {
     if (st->done) return;
     double d;
     if (seq) {
-        if (threadIdx.x < 64) {
-            const double t = seq_dot(a.sk, a.q, a.s, a.flag, a.vm);
-            if (threadIdx.x == 0) st->dot_tmp = t;
-        }
+        const double t = seq_dot(a.sk, a.q, a.s, a.flag, a.vm);
+        if (threadIdx.x == 0) st->dot_tmp = t;
         __syncthreads();
         d = st->dot_tmp;
     } else {
@@ -941,12 +941,10 @@ IFL_DEV void fin_sigma_body(const VecArgs& a, int n, int seq, int init, double t
     if (!init && st->done) return;
     double d;
     if (seq) {
-        if (threadIdx.x < 64) {
-            // inside the loop the reference's _z at a non-FLUID cell is A s
-            // (levels 4-5 take every cell; this product keeps A s in q)
-            const double t = seq_dot(a.sk, a.z, a.r, a.flag, a.vm, !init && a.pm && !a.vm ? a.q : nullptr);
-            if (threadIdx.x == 0) st->dot_tmp = t;
-        }
+        // inside the loop the reference's _z at a non-FLUID cell is A s
+        // (levels 4-5 take every cell; this product keeps A s in q)
+        const double t = seq_dot(a.sk, a.z, a.r, a.flag, a.vm, !init && a.pm && !a.vm ? a.q : nullptr);
+        if (threadIdx.x == 0) st->dot_tmp = t;
         __syncthreads();
         d = st->dot_tmp;
     } else {
@@ -1016,10 +1014,8 @@ __global__ void dot_partial_kernel(VecArgs a, const double* u, const double* v) 
 __global__ __launch_bounds__(256) void dot_finish_kernel(VecArgs a, const double* u, const double* v, int n, int seq, double* out) {
     double d;
     if (seq) {
-        if (threadIdx.x < 64) {
-            const double t = seq_dot(a.sk, u, v, a.flag, a.vm);
-            if (threadIdx.x == 0) *out = t;
-        }
+        const double t = seq_dot(a.sk, u, v, a.flag, a.vm);
+        if (threadIdx.x == 0) *out = t;
         return;
     }
     d = block_sum_partials(a.partial, n);
@@ -1070,15 +1066,15 @@ static int ring_setup(Solver* s, StreamArgs& a, size_t* shmem) {
     return IFL_OK;
 }
 
-template <int KIND, int MODE, int LV, int NG, typename T>
-static int launch_stream(Solver* s, StreamArgs a) {
+template <int KIND, int MODE, int LV, int NG, typename T, bool TR>
+static int launch_stream_k(Solver* s, StreamArgs a) {
     size_t shmem = 0;
     IFL_TRY((ring_setup<KIND, MODE, LV, NG, T>(s, a, &shmem)));
     // once per kernel; solvers stepped from several host threads may race
     // here, which only repeats the same call
     static std::atomic<bool> attr_set{false};
     if (!attr_set.load(std::memory_order_acquire)) {
-        IFL_HIP(hipFuncSetAttribute((const void*)stream_sweep_kernel<KIND, MODE, LV, NG, T>,
+        IFL_HIP(hipFuncSetAttribute((const void*)stream_sweep_kernel<KIND, MODE, LV, NG, T, TR>,
                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)shmem));
         attr_set.store(true, std::memory_order_release);
     }
@@ -1089,10 +1085,16 @@ static int launch_stream(Solver* s, StreamArgs a) {
     for (int done = 0; done < ns; done += chunk) {
         a.nlaunch = std::min(chunk, ns - done);
         a.strip0 = KIND == SW_BWD ? ns - done - a.nlaunch : done;
-        stream_sweep_kernel<KIND, MODE, LV, NG, T><<<sweep_blocks(a.nlaunch), 64 * kWaves, shmem, s->stream>>>(a);
+        stream_sweep_kernel<KIND, MODE, LV, NG, T, TR><<<sweep_blocks(a.nlaunch), 64 * kWaves, shmem, s->stream>>>(a);
         IFL_LAUNCH_CHECK();
     }
     return IFL_OK;
+}
+
+template <int KIND, int MODE, int LV, int NG, typename T>
+static int launch_stream(Solver* s, StreamArgs a) {
+    if (a.trace) return launch_stream_k<KIND, MODE, LV, NG, T, true>(s, a);
+    return launch_stream_k<KIND, MODE, LV, NG, T, false>(s, a);
 }
 
 static bool mixed(const Solver* s) { return s->cfg.precision == IFL_PREC_MIXED; }
@@ -1410,7 +1412,7 @@ template <int KIND, int LV, int NG>
 __global__ __launch_bounds__(64 * kWaves) void batch_sweep_kernel(const StreamArgs* t, const int2* map) {
     const int2 m = map[blockIdx.x];
     const StreamArgs a = t[m.x];
-    sweep_body<KIND, 1, LV, NG, double>(a, m.y);
+    sweep_body<KIND, 1, LV, NG, double, false>(a, m.y);
 }
 
 __global__ void batch_gather_kernel(PcgState* const* st, int n, PcgState* out) {
