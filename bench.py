@@ -49,7 +49,7 @@ def parse():
     ap.add_argument("--instances", type=int, default=256, help="batch: number of instances")
     ap.add_argument("--batch-mode", choices=["fused", "streams"], default="fused",
                     help="batch: one ifl_batch (PCG iterations fused over the instances) or one stream per instance")
-    ap.add_argument("--form", choices=["fp64", "mixed"], default=None,
+    ap.add_argument("--form", choices=["fp64", "mixed", "refine"], default=None,
                     help="time only this form (default: FP64 as the value, the mixed form beside it)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-iters", type=int, default=30)
@@ -175,7 +175,7 @@ def run_form(form, n, steps, warmup):
     import ifl_fluids as F
     import ifl_scenes as S
     scene = S.config3_scene()
-    prec = F.PREC_MIXED if form == "mixed" else F.PREC_FP64
+    prec = {"fp64": F.PREC_FP64, "mixed": F.PREC_MIXED, "refine": F.PREC_REFINE}[form]
     objs = [F.SolidBox(*b[1:]) if b[0] == S.BOX else F.SolidSphere(b[1], b[2], b[3], *b[5:]) for b in scene["bodies"]]
     solver = F.FluidSolver(7, n, n, scene["params"], objs, precision=prec)
     S.drive(solver, scene, warmup)
@@ -216,6 +216,49 @@ def run_form(form, n, steps, warmup):
                 "ms_per_iteration": prof["ms"] / max(1, pcg_iters), "bytes_per_cell_iteration": bpc,
                 "achieved_GBps": bpc * prof["cell_iterations"] / (prof["ms"] * 1e-3) / 1e9 if prof["ms"] > 0 else 0.0},
     }
+
+
+def mixed_frame_errors(n, frames=2, warm=2, form="mixed"):
+    """A reduced-precision form's error per frame against the FP64 form,
+    teacher-forced: from the FP64 form's state, one update() in each form, max relative
+    difference per field (d, t, u, v; p less its mean over FLUID cells,
+    since the pressure of these closed-wall systems is fixed only up to a
+    constant, which the FP64 iteration lets drift and the mixed form
+    deflates)."""
+    import numpy as np
+    import ifl_fluids as F
+    import ifl_scenes as S
+    scene = S.config3_scene()
+    objs = [F.SolidBox(*b[1:]) for b in scene["bodies"]]
+    ref = F.FluidSolver(7, n, n, scene["params"], objs)
+    mix = F.FluidSolver(7, n, n, scene["params"], objs,
+                        precision=F.PREC_MIXED if form == "mixed" else F.PREC_REFINE)
+    S.drive(ref, scene, warm)
+    fields = F.state_fields(7)
+    names = {"d": F.Q_D * 16, "t": F.Q_T * 16, "u": F.Q_U * 16, "v": F.Q_V * 16, "p": F.F_P}
+    out = []
+    for _ in range(frames):
+        for f in fields:
+            v = ref.field(f)
+            if v is not None:
+                mix.set_field(f, v)
+        for sv in (ref, mix):
+            for inf in scene["inflows"]:
+                sv.add_inflow(*inf)
+        sts = [sv.update(scene["dt"]) for sv in (ref, mix)]
+        fl = ref.field(F.Q_D * 16 + F.QF_CELL) == 0
+        err = {}
+        for k, f in names.items():
+            a, b = ref.field(f), mix.field(f)
+            if k == "p":
+                a = np.where(fl, a - a[fl].mean(), 0.0)
+                b = np.where(fl, b - b[fl].mean(), 0.0)
+            err[k] = float(np.abs(a - b).max() / max(np.abs(a).max(), 1e-300))
+        err["pressure_iterations"] = [st.pressure.iterations for st in sts]
+        out.append(err)
+    ref.close()
+    mix.close()
+    return out
 
 
 # Algorithmic bytes of the particle kernels: per particle (P) and per grid
@@ -466,7 +509,7 @@ def main():
     # the headline value is the form asked for (FP64 by default); without
     # --form the mixed form is run too and reported beside it
     main_form = args.form or "fp64"
-    forms = [main_form] if args.form else ["fp64", "mixed"]
+    forms = [main_form] if args.form else ["fp64", "mixed", "refine"]
     res = {f: run_form(f, n, args.steps, args.warmup) for f in forms}
     r = res[main_form]
     kern = r["kernels"]
@@ -502,7 +545,12 @@ def main():
                             "pcg_iterations_per_step": res[f]["mean_iters"],
                             "ms_per_pcg_iteration": res[f]["pcg"]["ms_per_iteration"],
                             "sweep_us": {k: v["avg_us"] for k, v in res[f]["kernels"].items()}} for f in forms}
-        out["forms"]["mixed_over_fp64"] = res["mixed"]["steps_per_s"] / res["fp64"]["steps_per_s"]
+        for f in forms[1:]:
+            out["forms"][f"{f}_over_fp64"] = res[f]["steps_per_s"] / res["fp64"]["steps_per_s"]
+            try:
+                out["forms"][f"{f}_frame_error"] = mixed_frame_errors(n, form=f)
+            except Exception as e:  # reported beside the measurement, never in place of it
+                out["forms"][f"{f}_frame_error"] = {"error": str(e)}
     if not args.no_cpu_baseline:
         try:
             out["cpu_baseline"] = cpu_baseline(r["scene"], n, r["mean_iters"])

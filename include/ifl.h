@@ -46,7 +46,7 @@ enum { IFL_CELL_FLUID = 0, IFL_CELL_SOLID = 1, IFL_CELL_EMPTY = 2 };
 enum { IFL_BODY_BOX = 0, IFL_BODY_SPHERE = 1, IFL_BODY_HOST = 2 };
 
 typedef struct ifl_body {
-    int32_t type;          /* IFL_BODY_BOX or IFL_BODY_SPHERE */
+    int32_t type;          /* IFL_BODY_BOX, IFL_BODY_SPHERE or IFL_BODY_HOST */
     int32_t reserved;
     double pos_x, pos_y;   /* SolidBody::_posX/_posY, 7:111-112 */
     double scale_x, scale_y;
@@ -60,11 +60,22 @@ typedef struct ifl_body {
 
 /* ---- configuration ---- */
 /* IFL_PREC_FP64: the drop-in, every operation in double as the reference.
- * IFL_PREC_MIXED: the same FP64 conjugate-gradient iteration (residual,
- * iterate, directions, A*s, dot products, alpha/beta and the convergence test
- * on max|r| in double), with the MIC(0) preconditioner applied in single
- * precision from a single-precision copy of the double-built factor. */
-enum { IFL_PREC_FP64 = 0, IFL_PREC_MIXED = 1 };
+ * IFL_PREC_MIXED: advection interpolated in single precision (trace
+ * positions in double); project() keeps the conjugate-gradient iteration in
+ * double (residual, iterate, directions, A*s, dot products, alpha/beta and
+ * the test max|r| < tol) and applies the MIC(0) preconditioner in single
+ * precision from a single-precision copy of the double-built factor;
+ * pressure solves apply it as P M^-1 P, P taking off the mean over FLUID
+ * cells (the pressure matrix is singular with closed walls, and the
+ * constant mode is left out of p).
+ * IFL_PREC_REFINE: advection as IFL_PREC_MIXED; project() as defect
+ * correction in double around a conjugate-gradient iteration entirely in
+ * single precision (factor, matrix and vectors in float; dot products,
+ * alpha and beta in double), the double residual updated by r -= A e after
+ * each inner solve and ended by the reference's test. It converges while
+ * the system's condition number times single-precision epsilon stays well
+ * below 1 (grids to ~512^2 for the pressure systems of solver 7). */
+enum { IFL_PREC_FP64 = 0, IFL_PREC_MIXED = 1, IFL_PREC_REFINE = 2 };
 enum { IFL_DOT_TREE = 0, IFL_DOT_SEQUENTIAL = 1 };
 
 typedef struct ifl_config {

@@ -227,6 +227,7 @@ void free_all(Solver* s) {
     F(s->r); F(s->p); F(s->z); F(s->s); F(s->precon); F(s->adiag); F(s->aplusx); F(s->aplusy);
     F(s->cx); F(s->cy); F(s->yv); F(s->qv); F(s->dflag); F(s->fmask); F(s->edge);
     F(s->cx32); F(s->cy32); F(s->pc32); F(s->y32); F(s->partial); F(s->st);
+    F(s->e32); F(s->r32); F(s->s32); F(s->z32); F(s->q32); F(s->ad32); F(s->apx32); F(s->apy32); F(s->st_outer);
     F(s->stage_buf); F(s->front); F(s->front_cnt); F(s->bodies_d); F(s->hreq); F(s->hreq_cnt);
     F(s->posx); F(s->posy);
     for (int t = 0; t < 4; t++) F(s->prop[t]);
@@ -300,12 +301,15 @@ int init_solver(Solver* s) {
                        &s->cx, &s->cy, &s->yv, &s->qv};
     for (double** a : arrs) IFL_TRY(dalloc(s, a, n));
     IFL_TRY(balloc(s, &s->dflag, n));
-    if (s->cfg.precision == IFL_PREC_MIXED) {
-        float** sarrs[] = {&s->cx32, &s->cy32, &s->pc32, &s->y32};
+    if (s->cfg.precision != IFL_PREC_FP64) {
+        float** sarrs[] = {&s->cx32, &s->cy32, &s->pc32, &s->y32, &s->e32, &s->r32, &s->s32,
+                           &s->z32, &s->q32, &s->ad32, &s->apx32, &s->apy32};
         for (float** a : sarrs) {
             IFL_HIP(hipMalloc((void**)a, n * sizeof(float)));
             IFL_HIP(hipMemsetAsync(*a, 0, n * sizeof(float), s->stream));
         }
+        IFL_HIP(hipMalloc((void**)&s->st_outer, sizeof(PcgState)));
+        IFL_HIP(hipMemsetAsync(s->st_outer, 0, sizeof(PcgState), s->stream));
     }
     IFL_HIP(hipMalloc((void**)&s->fmask, (size_t)sk.nstrips * sk.cols * sizeof(uint64_t)));
     IFL_HIP(hipMemsetAsync(s->fmask, 0, (size_t)sk.nstrips * sk.cols * sizeof(uint64_t), s->stream));
@@ -491,7 +495,9 @@ int update_grid(Solver* s, double dt, ifl_step_stats* stats) {
     while (true) {
         IFL_TRY(update_part(s, after, dt, &next));
         if (next == SOLVE_NONE) break;
+        s->solve_kind = next;
         IFL_TRY(solve(s, next == SOLVE_HEAT ? &heat : &press));
+        s->solve_kind = SOLVE_PRESSURE;
         after = next;
     }
     return finish_step(s, heat, press, stats);
@@ -633,8 +639,8 @@ int ifl_create(const ifl_config* cfg, ifl_solver** out) {
         g_err = "ifl_create: level 8 keeps 12 particles per cell; w*h*12 must fit in an int";
         return IFL_ERR_ARG;
     }
-    if (cfg->precision != IFL_PREC_FP64 && cfg->precision != IFL_PREC_MIXED) {
-        g_err = "ifl_create: precision must be IFL_PREC_FP64 or IFL_PREC_MIXED";
+    if (cfg->precision != IFL_PREC_FP64 && cfg->precision != IFL_PREC_MIXED && cfg->precision != IFL_PREC_REFINE) {
+        g_err = "ifl_create: precision must be IFL_PREC_FP64, IFL_PREC_MIXED or IFL_PREC_REFINE";
         return IFL_ERR_ARG;
     }
     int ndev = 0;

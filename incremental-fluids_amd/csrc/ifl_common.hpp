@@ -248,6 +248,47 @@ IFL_DEV double qcerp(const double* __restrict__ src, int w, int h, double ox, do
     return cerp1(qv[0], qv[1], qv[2], qv[3], y);
 }
 
+// The same samplers with the interpolation in single precision (the mixed
+// form's advection): the sample point is located in double, the double
+// fields are read and rounded to float, and the weights and sums are float.
+IFL_DEV float lerp1f(float a, float b, float x) { return a * (1.0f - x) + b * x; }
+IFL_DEV float cerp1f(float a, float b, float c, float d, float x) {
+    float xsq = x * x;
+    float xcu = xsq * x;
+    float minv = fminf(a, fminf(b, fminf(c, d)));
+    float maxv = fmaxf(a, fmaxf(b, fmaxf(c, d)));
+    float t = a * (0.0f - 0.5f * x + 1.0f * xsq - 0.5f * xcu) + b * (1.0f + 0.0f * x - 2.5f * xsq + 1.5f * xcu) +
+              c * (0.0f + 0.5f * x + 2.0f * xsq - 1.5f * xcu) + d * (0.0f + 0.0f * x - 0.5f * xsq + 0.5f * xcu);
+    return fminf(fmaxf(t, minv), maxv);
+}
+IFL_DEV float qlerpf(const double* __restrict__ src, int w, int h, double ox, double oy, double x, double y) {
+    x = smin(smax(x - ox, 0.0), w - 1.001);
+    y = smin(smax(y - oy, 0.0), h - 1.001);
+    int ix = trunc_idx(x);
+    int iy = trunc_idx(y);
+    const float fx = (float)(x - ix), fy = (float)(y - iy);
+    const double* r0 = src + (size_t)iy * w + ix;
+    float x00 = (float)r0[0], x10 = (float)r0[1];
+    float x01 = (float)r0[w], x11 = (float)r0[w + 1];
+    return lerp1f(lerp1f(x00, x10, fx), lerp1f(x01, x11, fx), fy);
+}
+IFL_DEV float qcerpf(const double* __restrict__ src, int w, int h, double ox, double oy, double x, double y) {
+    x = smin(smax(x - ox, 0.0), w - 1.001);
+    y = smin(smax(y - oy, 0.0), h - 1.001);
+    int ix = trunc_idx(x);
+    int iy = trunc_idx(y);
+    const float fx = (float)(x - ix), fy = (float)(y - iy);
+    int x0 = imax(ix - 1, 0), x1 = ix, x2 = ix + 1, x3 = imin(ix + 2, w - 1);
+    int ys[4] = {imax(iy - 1, 0), iy, iy + 1, imin(iy + 2, h - 1)};
+    float qv[4];
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        const double* row = src + (size_t)ys[k] * w;
+        qv[k] = cerp1f((float)row[x0], (float)row[x1], (float)row[x2], (float)row[x3], fx);
+    }
+    return cerp1f(qv[0], qv[1], qv[2], qv[3], fy);
+}
+
 // ---------------------------------------------------------------- PCG layout
 // The PCG vectors and matrix live in a strip-skewed layout: rows are grouped
 // in strips of 64 (one wave, one lane per row); inside a strip, cell

@@ -29,6 +29,8 @@ struct PcgState {
     int printed_iter;  // the `iter` the reference prints
     int timeout;       // a bounded wait gave up (kernel bug, never expected)
     int pad[3];
+    double zmean;      // mixed form, pressure solves: the mean over FLUID cells taken off z
+    double rmean;      // mixed form, pressure solves: the mean over FLUID cells taken off r
 };
 
 struct Profile {
@@ -58,8 +60,15 @@ struct Solver {
     double *precon = nullptr, *adiag = nullptr, *aplusx = nullptr, *aplusy = nullptr;
     double *cx = nullptr, *cy = nullptr;  // aPlusX*precon, aPlusY*precon (the sweeps' products)
     double *yv = nullptr;                 // forward-sweep output
-    // mixed form: the factor and the forward-sweep output in single precision
+    // mixed form: the factor and the forward-sweep output in single precision,
+    // and the inner conjugate-gradient iteration's vectors (correction e,
+    // residual, directions, preconditioned residual, A s) and matrix
     float *cx32 = nullptr, *cy32 = nullptr, *pc32 = nullptr, *y32 = nullptr;
+    float *e32 = nullptr, *r32 = nullptr, *s32 = nullptr, *z32 = nullptr, *q32 = nullptr;
+    float *ad32 = nullptr, *apx32 = nullptr, *apy32 = nullptr;
+    PcgState* st_outer = nullptr;  // the outer (double) residual's norm
+    int solve_kind = 2;            // SOLVE_HEAT or SOLVE_PRESSURE: the solve project() runs
+    long nfluid = 0;               // FLUID cells of d (the mixed form's deflation)
     double *qv = nullptr;                 // A*s
     uint8_t* dflag = nullptr;             // 1 where the d-cell is FLUID (and inside the grid)
     uint64_t* fmask = nullptr;            // dflag as one 64-lane mask per strip column (nstrips*cols)

@@ -25,6 +25,7 @@
 // the reference order).
 #include <algorithm>
 #include <atomic>
+#include <type_traits>
 #include <vector>
 
 #include "ifl_solver.hpp"
@@ -210,16 +211,19 @@ IFL_DEV float lane_shl1(float v, float edge) { return wave_shl1f(v, edge); }
 // Layout of one sweep. T is the precision of the triangular solves: double
 // in the drop-in, float in the mixed form (the factor build is always
 // double). NG arrays are streamed from HBM; NS of them are kept in the LDS
-// slot (the forward update folds q into r on the way). The residual and the
-// vectors the writer reduces with stay double; the factor (cx, cy, precon)
-// and the forward result y are T. Each array occupies 512*es bytes of a
-// slot, followed by wave 0's output (512 elements).
+// slot (the forward update folds q into r on the way). MODE 0 is the stage
+// form and MODE 1 the PCG loop, where the residual and the vectors the writer
+// reduces with stay double and the factor (cx, cy, precon) and the forward
+// result y are T; MODE 2 is the mixed form's inner PCG loop, where every
+// vector is T too. Each array occupies 512*es bytes of a slot, followed by
+// wave 0's output (512 elements).
 template <int KIND, int MODE, int NG, typename T>
 struct SweepShape {
-    static constexpr int NS = NG - (KIND == SW_FWD && MODE == 1 ? 1 : 0);
+    static constexpr int NS = NG - (KIND == SW_FWD && MODE >= 1 ? 1 : 0);
     static constexpr int es(int ai) {
-        return KIND == SW_FWD ? ((ai == 0 || ai == 4) ? 8 : (int)sizeof(T))
-                              : (KIND == SW_BWD ? (ai <= 3 ? (int)sizeof(T) : 8) : 8);
+        return (MODE == 2 && KIND != SW_PRECON) ? (int)sizeof(T)
+               : KIND == SW_FWD ? ((ai == 0 || ai == 4) ? 8 : (int)sizeof(T))
+                                : (KIND == SW_BWD ? (ai <= 3 ? (int)sizeof(T) : 8) : 8);
     }
     static constexpr int rb(int ai) { return 512 * es(ai); }
     static constexpr int ro(int ai) { return ai == 0 ? 0 : ro(ai - 1) + rb(ai - 1); }
@@ -255,10 +259,11 @@ template <int KIND, int MODE, int LV, int NG, typename T>
 IFL_DEV void loader_wave(const StreamArgs& a, char* ring, RingHdr* hdr, int k, int lane, long sbase, int nblk, int j) {
     typedef SweepShape<KIND, MODE, NG, T> S;
     constexpr int R = S::R, NL = S::NLD;
-    constexpr bool UPD = KIND == SW_FWD && MODE == 1;
+    constexpr bool UPD = KIND == SW_FWD && MODE >= 1;
     const int cols = a.sk.cols;
     const int nmine = nblk > k ? (nblk - k + kLoaders - 1) / kLoaders : 0;
     const double alpha_neg = UPD ? -a.st->alpha : 0.0;
+    const float alpha_neg_f = (float)alpha_neg;  // the mixed form's inner loop
     const const_u64* fm = strip_masks(a, j);
     vec4i b0[NL], b1[NL], b2[NL];
     // the FLUID masks the residual update needs (levels 6+), loaded with the
@@ -274,17 +279,22 @@ IFL_DEV void loader_wave(const StreamArgs& a, char* ring, RingHdr* hdr, int k, i
 #pragma unroll
             for (int m = 0; m < S::rb(ai) / 1024; m++) B[S::lo(ai) + m] = __builtin_nontemporal_load((const vec4i*)(p + m * 1024));
         }
-        if (MASKED) {
+        if (MASKED && S::es(0) == 8) {
             // element 2*lane + c of run m is step 2m + lane/32, row 2*(lane%32) + c
 #pragma unroll
             for (int m = 0; m < 4; m++) WM[m] = fm[col0 + 2 * m + (lane >> 5)];
+        } else if (MASKED) {
+            // single precision: element 4*lane + c of run m is step 4m + lane/16,
+            // row 4*(lane%16) + c
+#pragma unroll
+            for (int m = 0; m < 2; m++) WM[m] = fm[col0 + 4 * m + (lane >> 4)];
         }
     };
     // land block i (from B) in its slot, then reuse B for block i + R
     auto stage = [&](int i, vec4i* B, uint64_t* WM) __attribute__((always_inline)) -> bool {
         if (i >= nmine) return true;
         const int b = k + i * kLoaders;
-        if (UPD) {
+        if (UPD && S::es(0) == 8) {
 #pragma unroll
             for (int m = 0; m < 4; m++) {
                 const vec2d r = __builtin_bit_cast(vec2d, B[m]);
@@ -293,6 +303,21 @@ IFL_DEV void loader_wave(const StreamArgs& a, char* ring, RingHdr* hdr, int k, i
                     const unsigned bits = (unsigned)(WM[m] >> (2 * (lane & 31)));
                     rn.x = (bits & 1u) ? rn.x : r.x;
                     rn.y = (bits & 2u) ? rn.y : r.y;
+                }
+                B[m] = __builtin_bit_cast(vec4i, rn);
+            }
+        } else if (UPD) {
+            typedef float vec4f __attribute__((ext_vector_type(4)));
+#pragma unroll
+            for (int m = 0; m < 2; m++) {
+                const vec4f r = __builtin_bit_cast(vec4f, B[m]);
+                vec4f rn = r + __builtin_bit_cast(vec4f, B[S::lo(4) + m]) * alpha_neg_f;
+                if (MASKED) {
+                    const unsigned bits = (unsigned)(WM[m] >> (4 * (lane & 15)));
+                    rn.x = (bits & 1u) ? rn.x : r.x;
+                    rn.y = (bits & 2u) ? rn.y : r.y;
+                    rn.z = (bits & 4u) ? rn.z : r.z;
+                    rn.w = (bits & 8u) ? rn.w : r.w;
                 }
                 B[m] = __builtin_bit_cast(vec4i, rn);
             }
@@ -426,9 +451,13 @@ IFL_DEV void compute_wave(const StreamArgs& a, char* ring, RingHdr* hdr, const d
     typedef const __attribute__((address_space(4))) vec4u const_u64x4;
     constexpr int NP = kBlk / 2;
     V2 XA[NT][NP], XB[NT][NP];
-    vec2d QA[NP], QB[NP], EA[NP], EB[NP];
+    // the updated residual: double, or T in the mixed form's inner loop
+    typedef typename std::conditional<MODE == 2, T, double>::type QE;
+    typedef typename std::conditional<MODE == 2, V2, vec2d>::type QV;
+    QV QA[NP], QB[NP];
+    vec2d EA[NP], EB[NP];
     vec4u MA0, MA1, MB0, MB1;
-    auto load_blk = [&](const char* slot, int b, V2 (&X)[NT][NP], vec2d (&Q)[NP], vec2d (&E)[NP], vec4u& M0, vec4u& M1) {
+    auto load_blk = [&](const char* slot, int b, V2 (&X)[NT][NP], QV (&Q)[NP], vec2d (&E)[NP], vec4u& M0, vec4u& M1) {
         const int bm = b < nblk ? b : nblk - 1;
         // a block's 8 edges are contiguous in the ring (8 divides its size)
         const double* eb = ering + ((unsigned)(bm * kBlk) & (unsigned)(kERing - 1));
@@ -443,7 +472,7 @@ IFL_DEV void compute_wave(const StreamArgs& a, char* ring, RingHdr* hdr, const d
                 X[i][p].y = arr[e1];
             }
             if (KIND == SW_FWD) {
-                const double* arr = (const double*)slot;
+                const QE* arr = (const QE*)slot;
                 Q[p].x = arr[e0];
                 Q[p].y = arr[e1];
             }
@@ -460,8 +489,8 @@ IFL_DEV void compute_wave(const StreamArgs& a, char* ring, RingHdr* hdr, const d
         return c == 0 ? M.x : c == 1 ? M.y : c == 2 ? M.z : M.w;
     };
     T Pw = 0, Pn = 0, zs = 0;
-    auto run_blk = [&](const char* slot, int b, const V2 (&X)[NT][NP], const vec2d (&Q)[NP], const vec2d (&E)[NP],
-                       const vec4u& M0, const vec4u& M1, const char* nslot, V2 (&Y)[NT][NP], vec2d (&P)[NP],
+    auto run_blk = [&](const char* slot, int b, const V2 (&X)[NT][NP], const QV (&Q)[NP], const vec2d (&E)[NP],
+                       const vec4u& M0, const vec4u& M1, const char* nslot, V2 (&Y)[NT][NP], QV (&P)[NP],
                        vec2d (&F)[NP], vec4u& N0, vec4u& N1) {
         T* out = (T*)(slot + S::OUT);
         __builtin_amdgcn_s_waitcnt(0xC07F);  // lgkmcnt(0)
@@ -555,7 +584,10 @@ IFL_DEV void writer_wave(const StreamArgs& a, char* ring, RingHdr* hdr, int j, i
     const long sbase = (long)j * a.sk.strip_sz;
     const bool row_ok = j * kStrip + lane < H;
     const double tau = a.tau;
-    double rmax = 0.0, acc = 0.0;
+    // the mixed form's pressure solves keep r less its mean over FLUID cells;
+    // the convergence test reads r itself
+    const double r0 = KIND == SW_FWD && sizeof(T) == 4 ? a.st->rmean : 0.0;
+    double rmax = 0.0, acc = 0.0, zsum = 0.0;
     for (int b = 0; b < nblk; b++) {
         if (!lds_wait_gt(&hdr->computed, b, hdr, a.timeout)) return;
         char* slot = ring + (long)(b % a.nb) * a.slot_bytes;
@@ -621,8 +653,8 @@ IFL_DEV void writer_wave(const StreamArgs& a, char* ring, RingHdr* hdr, int j, i
             const bool take = VM ? f : valid;
             if (KIND == SW_FWD) {
                 // infinityNorm(_r) 7:871 (masked in 6+) of the residual the sweep used
-                const double rv = ((const double*)in(0))[e];
-                const double m = smax(rmax, fabs(rv));
+                const double rv = MODE == 2 ? (double)((const T*)in(0))[e] : ((const double*)in(0))[e];
+                const double m = smax(rmax, fabs(sizeof(T) == 4 ? rv + r0 : rv));
                 rmax = take ? m : rmax;
             } else if (KIND == SW_BWD) {
                 // _z (wave 0 left +0 at non-FLUID cells); dotProduct(_z, _r)
@@ -632,16 +664,19 @@ IFL_DEV void writer_wave(const StreamArgs& a, char* ring, RingHdr* hdr, int j, i
                 if (MODE == 0) {
                     // the stage form keeps _z at non-FLUID cells as it was
                     if (f) ((double*)a.out[0])[gblk + e] = zz;
-                } else if (sizeof(T) != 8) {
+                } else if (MODE == 1 && sizeof(T) != 8) {
                     // inside the PCG loop _z at non-FLUID cells is not read
                     // (levels 4-5 take A s from q; the exit restores it), so
                     // every cell is stored; the mixed form widens on the way
                     ((double*)a.out[0])[gblk + e] = zz;
                 }
-                const double rv = ((const double*)in(4))[e];
-                const double zdot = f ? zz : (AUX ? ((const double*)in(5))[e] : 0.0);
+                const double rv = MODE == 2 ? (double)((const T*)in(4))[e] : ((const double*)in(4))[e];
+                const double aux = !AUX ? 0.0 : MODE == 2 ? (double)((const T*)in(5))[e] : ((const double*)in(5))[e];
+                const double zdot = f ? zz : aux;
                 const double na = acc + zdot * rv;
                 acc = take ? na : acc;
+                if (MODE == 2 || sizeof(T) == 4)
+                    zsum = f ? zsum + zz : zsum;  // the mean taken off z (deflation)
             } else {
                 // the factor at FLUID cells; cx = aPlusX*precon, cy likewise, +0 elsewhere
                 const double pcv = (double)out[e];
@@ -667,9 +702,9 @@ IFL_DEV void writer_wave(const StreamArgs& a, char* ring, RingHdr* hdr, int j, i
         };
         if (KIND == SW_FWD) {
             store_block(a.out[0], (const char*)out, sizeof(T));
-            if (MODE == 1) store_block(a.out[1], in(0), 8);
+            if (MODE >= 1) store_block(a.out[1], in(0), S::es(0));
         } else if (KIND == SW_BWD) {
-            if (MODE == 1 && sizeof(T) == 8) store_block(a.out[0], (const char*)out, 8);
+            if ((MODE == 1 && sizeof(T) == 8) || MODE == 2) store_block(a.out[0], (const char*)out, sizeof(T));
         } else if (KIND == SW_PRECON) {
             store_block(a.out[1], in(1), 8);
             store_block(a.out[2], in(2), 8);
@@ -683,6 +718,10 @@ IFL_DEV void writer_wave(const StreamArgs& a, char* ring, RingHdr* hdr, int j, i
     } else if (KIND == SW_BWD) {
         acc = wave_sum(acc);
         if (lane == 0) a.partial[j] = acc;
+        if (MODE == 2 || sizeof(T) == 4) {
+            zsum = wave_sum(zsum);
+            if (lane == 0) a.partial[a.sk.nstrips + j] = zsum;
+        }
     }
 }
 
@@ -690,7 +729,7 @@ template <int KIND, int MODE, int LV, int NG, typename T, bool TR>
 IFL_DEV void sweep_body(const StreamArgs& a, int j) {
     extern __shared__ __align__(16) char smem[];
     PcgState* st = a.st;
-    if (KIND != SW_PRECON && ((MODE == 1 && st->done) || st->timeout)) return;
+    if (KIND != SW_PRECON && ((MODE >= 1 && st->done) || st->timeout)) return;
     RingHdr* hdr = (RingHdr*)smem;
     double* ering = (double*)(smem + sizeof(RingHdr));
     char* ring = smem + sizeof(RingHdr) + kERing * sizeof(double);
@@ -733,6 +772,7 @@ struct VecArgs {
     PcgState* st;
     int pm;  // levels 4+: the preconditioner writes _z at FLUID cells only
     int vm;  // levels 6+: vector operations are masked to FLUID cells
+    int defl;  // the mixed form's pressure solves: s is formed from z less its mean over FLUID cells
 };
 
 IFL_DEV bool skew_cell(const Skew& sk, long i, int& x, int& y) {
@@ -762,7 +802,11 @@ IFL_DEV void vec_update_body(const VecArgs& a, long blk, int mode, int check_don
     if (a.vm && !fl) return;
     double s = a.s[i];
     if (mode != 1) a.p[i] = a.p[i] + s * st->alpha;
-    if (mode != 2) a.s[i] = (a.pm && !fl ? a.q[i] : a.z[i]) + s * st->beta;
+    if (mode != 2) {
+        double zi = a.pm && !fl ? a.q[i] : a.z[i];
+        if (a.defl && fl) zi = zi - st->zmean;
+        a.s[i] = zi + s * st->beta;
+    }
 }
 __global__ void vec_update_kernel(VecArgs a, int mode, int check_done) { vec_update_body(a, blockIdx.x, mode, check_done); }
 
@@ -979,6 +1023,8 @@ __global__ __launch_bounds__(256) void fin_sigma_kernel(VecArgs a, int n, int se
 }
 
 __global__ void state_reset_kernel(PcgState* st) {
+    st->zmean = 0.0;
+    st->rmean = 0.0;
     st->sigma = st->alpha = st->beta = 0.0;
     st->max_err = 0.0;
     st->dot_tmp = 0.0;
@@ -1033,6 +1079,7 @@ static VecArgs vec_args(Solver* s) {
     a.st = s->st;
     a.pm = s->level >= 4;
     a.vm = s->level >= 6;
+    a.defl = s->cfg.precision == IFL_PREC_MIXED && s->solve_kind == SOLVE_PRESSURE;
     return a;
 }
 
@@ -1162,7 +1209,7 @@ static int launch_precon(Solver* s, long long* trace) {
     a.out[0] = s->precon;
     a.out[1] = s->cx;
     a.out[2] = s->cy;
-    if (mixed(s)) {
+    if (s->cfg.precision != IFL_PREC_FP64) {  // both reduced-precision forms sweep the float copies
         a.out[3] = s->pc32;
         a.out[4] = s->cx32;
         a.out[5] = s->cy32;
@@ -1227,6 +1274,209 @@ int k_inf_norm(Solver* s, double* out) {
     return IFL_OK;
 }
 
+struct VecArgs32 {
+    Skew sk;
+    float *e, *s, *z, *r, *q;
+    const float *adiag, *apx, *apy;
+    const uint8_t* flag;
+    double* partial;
+    PcgState* st;
+    int pm, vm;
+};
+
+// e += alpha*s, s = z + beta*s (mode 0); mode 2 only the e update
+__global__ void vec_update32_kernel(VecArgs32 a, int mode, int check_done) {
+    PcgState* st = a.st;
+    if (check_done && st->done) return;
+    long n = (long)a.sk.nstrips * a.sk.strip_sz;
+    long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    int x, y;
+    if (!skew_cell(a.sk, i, x, y)) return;
+    const bool fl = a.flag[i] != 0;
+    if (a.vm && !fl) return;
+    const float sv = a.s[i];
+    if (mode != 1) a.e[i] = a.e[i] + sv * (float)st->alpha;
+    if (mode != 2) a.s[i] = (a.pm && !fl ? a.q[i] : a.z[i] - (float)st->zmean) + sv * (float)st->beta;
+}
+
+// the first direction: s = z (less its mean over FLUID cells when deflating)
+__global__ void mixed_init_s_kernel(VecArgs32 a) {
+    long n = (long)a.sk.nstrips * a.sk.strip_sz;
+    long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float zm = a.flag[i] ? (float)a.st->zmean : 0.0f;
+    a.s[i] = a.z[i] - zm;
+}
+
+// sums over FLUID cells of r (double) and the FLUID count, per block
+__global__ __launch_bounds__(256) void mixed_rsum_kernel(VecArgs a, long n) {
+    __shared__ double red[4], cnt[4];
+    double v = 0.0, c = 0.0;
+    for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
+        int x, y;
+        if (skew_cell(a.sk, i, x, y) && a.flag[i]) {
+            v += a.r[i];
+            c += 1.0;
+        }
+    }
+    v = wave_sum(v);
+    c = wave_sum(c);
+    if ((threadIdx.x & 63) == 0) {
+        red[threadIdx.x >> 6] = v;
+        cnt[threadIdx.x >> 6] = c;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        a.partial[2 * blockIdx.x] = (red[0] + red[1]) + (red[2] + red[3]);
+        a.partial[2 * blockIdx.x + 1] = (cnt[0] + cnt[1]) + (cnt[2] + cnt[3]);
+    }
+}
+
+// mean of r over FLUID cells into outer->zmean (nb blocks of partial pairs),
+// and into st->rmean when st is given
+__global__ __launch_bounds__(256) void mixed_rmean_kernel(const double* partial, int nb, PcgState* outer,
+                                                          PcgState* st) {
+    __shared__ double red[256], cnt[256];
+    double v = 0.0, c = 0.0;
+    for (int k = threadIdx.x; k < nb; k += 256) {
+        v += partial[2 * k];
+        c += partial[2 * k + 1];
+    }
+    red[threadIdx.x] = v;
+    cnt[threadIdx.x] = c;
+    __syncthreads();
+    for (int off = 128; off >= 1; off >>= 1) {
+        if (threadIdx.x < off) {
+            red[threadIdx.x] += red[threadIdx.x + off];
+            cnt[threadIdx.x] += cnt[threadIdx.x + off];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        outer->zmean = cnt[0] > 0 ? red[0] / cnt[0] : 0.0;
+        outer->dot_tmp = cnt[0];
+        if (st) st->rmean = outer->zmean;
+    }
+}
+
+// finish C of the inner loop: with deflation, the mean of z over FLUID cells
+// (z's partial sums follow the z.r ones) before sigma and beta
+__global__ __launch_bounds__(256) void fin_sigma32_kernel(VecArgs a, int n, int init, double tol, int limit,
+                                                          int deflate, const PcgState* outer) {
+    PcgState* st = a.st;
+    if (!init && st->done) return;
+    if (deflate) {
+        const double zs = block_sum_partials(a.partial + n, n);
+        if (threadIdx.x == 0) st->zmean = zs / outer->dot_tmp;
+        __syncthreads();
+    }
+    fin_sigma_body(a, n, 0, init, tol, limit);
+}
+
+// q = A s in single precision (the matrixVectorProduct stencil), q.s partials in double
+__global__ __launch_bounds__(256) void matvec32_kernel(VecArgs32 a, int check_done) {
+    __shared__ double red[4];
+    PcgState* st = a.st;
+    if (check_done && st->done) return;
+    const Skew& sk = a.sk;
+    long n = (long)sk.nstrips * sk.strip_sz;
+    long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    double prod = 0.0;
+    int x, y;
+    if (i < n && skew_cell(sk, i, x, y)) {
+        const int l = y & 63;
+        const float* b = a.s;
+        float t = a.adiag[i] * b[i];
+        if (x > 0) t += a.apx[i - kStrip] * b[i - kStrip];
+        if (y > 0) {
+            long js = l > 0 ? i - kStrip - 1 : sk.pos(x, y - 1);
+            t += a.apy[js] * b[js];
+        }
+        if (x < sk.W - 1) t += a.apx[i] * b[i + kStrip];
+        if (y < sk.H - 1) {
+            long jn = l < kStrip - 1 ? i + kStrip + 1 : sk.pos(x, y + 1);
+            t += a.apy[i] * b[jn];
+        }
+        a.q[i] = t;
+        if (!a.vm || a.flag[i]) prod = (double)t * (double)b[i];
+    }
+    prod = wave_sum(prod);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = prod;
+    __syncthreads();
+    if (threadIdx.x == 0) a.partial[blockIdx.x] = (red[0] + red[1]) + (red[2] + red[3]);
+}
+
+// single-precision copies: the matrix (what = 0), or the start of a round:
+// r32 = r, e32 = 0, q32 = 0 (what = 1)
+__global__ void mixed_copy_kernel(VecArgs a, VecArgs32 b, int what) {
+    long n = (long)a.sk.nstrips * a.sk.strip_sz;
+    long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    if (what == 0) {
+        ((float*)b.adiag)[i] = (float)a.adiag[i];
+        ((float*)b.apx)[i] = (float)a.apx[i];
+        ((float*)b.apy)[i] = (float)a.apy[i];
+    } else {
+        // with deflation (pressure solves), r less its mean over FLUID cells
+        b.r[i] = (float)(a.flag[i] ? a.r[i] - a.st->zmean : a.r[i]);
+        b.e[i] = 0.0f;
+        b.q[i] = 0.0f;
+    }
+}
+
+// the end of a round, in double: p += e, r -= A e (the matrixVectorProduct
+// stencil on e), max|r| over the norm domain into st_outer
+__global__ __launch_bounds__(256) void mixed_outer_kernel(VecArgs a, const float* __restrict__ e32, PcgState* outer) {
+    const Skew& sk = a.sk;
+    long n = (long)sk.nstrips * sk.strip_sz;
+    long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    double m = 0.0;
+    int x, y;
+    if (i < n && skew_cell(sk, i, x, y)) {
+        const bool fl = a.flag[i] != 0;
+        const int l = y & 63;
+        const double ei = (double)e32[i];
+        double t = a.adiag[i] * ei;
+        if (x > 0) t += a.apx[i - kStrip] * (double)e32[i - kStrip];
+        if (y > 0) {
+            long js = l > 0 ? i - kStrip - 1 : sk.pos(x, y - 1);
+            t += a.apy[js] * (double)e32[js];
+        }
+        if (x < sk.W - 1) t += a.apx[i] * (double)e32[i + kStrip];
+        if (y < sk.H - 1) {
+            long jn = l < kStrip - 1 ? i + kStrip + 1 : sk.pos(x, y + 1);
+            t += a.apy[i] * (double)e32[jn];
+        }
+        if (!a.vm || fl) {
+            a.p[i] = a.p[i] + ei;
+            const double rn = a.r[i] - t;
+            a.r[i] = rn;
+            m = fabs(rn);
+        }
+    }
+    m = wave_max(m);
+    if ((threadIdx.x & 63) == 0) atomicMax((unsigned long long*)&outer->max_err, (unsigned long long)as_bits(m));
+}
+
+// r -= rmean (sign -1, the start of a deflated solve) or r += rmean (sign +1,
+// its end) at FLUID cells
+__global__ void defl_shift_kernel(VecArgs a, double sign) {
+    long n = (long)a.sk.nstrips * a.sk.strip_sz;
+    long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    if (i >= n || !a.flag[i]) return;
+    a.r[i] = a.r[i] + sign * a.st->rmean;
+}
+__global__ void rmean_clear_kernel(PcgState* st) { st->rmean = 0.0; }
+
+// the first direction of a deflated solve: s = z less its mean over FLUID cells (7:856)
+__global__ void defl_init_s_kernel(VecArgs a) {
+    long n = (long)a.sk.nstrips * a.sk.strip_sz;
+    long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    a.s[i] = a.flag[i] ? a.z[i] - a.st->zmean : a.z[i];
+}
+
 // Iteration `it` of the solve; each kernel returns at once when the solve is
 // already done.
 static int launch_iteration(Solver* s, int it, int limit) {
@@ -1252,13 +1502,254 @@ static int launch_iteration(Solver* s, int it, int limit) {
     IFL_TRY(prof_begin(s, IFL_KERNEL_SWEEP_BWD, it));
     IFL_TRY(launch_bwd<1>(s));
     IFL_TRY(prof_end(s));
-    fin_sigma_kernel<<<1, 256, 0, s->stream>>>(a, s->sk.nstrips, seq, 0, s->cfg.tol, limit);
+    if (a.defl)
+        fin_sigma32_kernel<<<1, 256, 0, s->stream>>>(a, s->sk.nstrips, 0, s->cfg.tol, limit, 1, s->st_outer);
+    else
+        fin_sigma_kernel<<<1, 256, 0, s->stream>>>(a, s->sk.nstrips, seq, 0, s->cfg.tol, limit);
     IFL_LAUNCH_CHECK();
+    return IFL_OK;
+}
+
+// ---------------------------------------------------------------- the mixed form
+// project() of the mixed form: defect correction in double around an inner
+// conjugate-gradient iteration in single precision. The outer loop keeps the
+// reference's double residual r and solution p; each round solves A e = r in
+// single precision (the MIC(0) factor, the matrix and every vector in float;
+// dot products, alpha and beta in double) until its residual has fallen by
+// kInnerDrop, then adds e to p and takes A e off r in double. The stopping
+// test is the reference's, max|r| < tol on that double residual (updated
+// recursively, as the reference's is), and the budget counts inner
+// iterations.
+#ifndef IFL_INNER_DROP
+#define IFL_INNER_DROP 1e-4
+#endif
+#ifndef IFL_MAX_OUTER
+#define IFL_MAX_OUTER 16
+#endif
+constexpr double kInnerDrop = IFL_INNER_DROP;
+constexpr int kMaxOuter = IFL_MAX_OUTER;
+
+static VecArgs32 vec_args32(Solver* s) {
+    VecArgs32 a;
+    a.sk = s->sk;
+    a.e = s->e32; a.s = s->s32; a.z = s->z32; a.r = s->r32; a.q = s->q32;
+    a.adiag = s->ad32; a.apx = s->apx32; a.apy = s->apy32;
+    a.flag = s->dflag;
+    a.partial = s->partial;
+    a.st = s->st;
+    a.pm = s->level >= 4;
+    a.vm = s->level >= 6;
+    return a;
+}
+
+static StreamArgs fwd_args2(Solver* s) {
+    StreamArgs a = stream_args(s, SW_FWD);
+    a.in[0] = s->r32;
+    a.in[1] = s->cx32;
+    a.in[2] = s->cy32;
+    a.in[3] = s->pc32;
+    a.in[4] = s->q32;
+    a.out[0] = s->y32;
+    a.out[1] = s->r32;
+    return a;
+}
+static StreamArgs bwd_args2(Solver* s) {
+    StreamArgs a = stream_args(s, SW_BWD);
+    a.in[0] = s->y32;
+    a.in[1] = s->cx32;
+    a.in[2] = s->cy32;
+    a.in[3] = s->pc32;
+    a.in[4] = s->r32;
+    a.in[5] = s->q32;
+    a.out[0] = s->z32;
+    return a;
+}
+static int launch_fwd2(Solver* s) {
+    StreamArgs a = fwd_args2(s);
+    if (s->level == 3) return launch_stream<SW_FWD, 2, 3, 5, float>(s, a);
+    if (s->level <= 5) return launch_stream<SW_FWD, 2, 4, 5, float>(s, a);
+    return launch_stream<SW_FWD, 2, 6, 5, float>(s, a);
+}
+static int launch_bwd2(Solver* s) {
+    StreamArgs a = bwd_args2(s);
+    if (s->level == 3) return launch_stream<SW_BWD, 2, 3, 5, float>(s, a);
+    if (s->level <= 5) return launch_stream<SW_BWD, 2, 4, 6, float>(s, a);
+    return launch_stream<SW_BWD, 2, 6, 5, float>(s, a);
+}
+
+// one inner iteration (the phases of launch_iteration on the single-precision vectors)
+static int launch_iteration32(Solver* s, int it, int limit, double tol, int gidx, int deflate) {
+    VecArgs a = vec_args(s);
+    VecArgs32 b = vec_args32(s);
+    long n = skew_n(s);
+    int nbv = grid_blocks(n, 256);
+    if (it > 0) {
+        vec_update32_kernel<<<nbv, 256, 0, s->stream>>>(b, 0, 1);
+        IFL_LAUNCH_CHECK();
+    }
+    IFL_TRY(prof_begin(s, IFL_KERNEL_MATVEC, gidx));
+    matvec32_kernel<<<nbv, 256, 0, s->stream>>>(b, 1);
+    IFL_LAUNCH_CHECK();
+    IFL_TRY(prof_end(s));
+    fin_alpha_kernel<<<1, 256, 0, s->stream>>>(a, nbv, 0);
+    IFL_LAUNCH_CHECK();
+    IFL_TRY(prof_begin(s, IFL_KERNEL_SWEEP_FWD, gidx));
+    IFL_TRY(launch_fwd2(s));
+    IFL_TRY(prof_end(s));
+    fin_check_kernel<<<1, 1, 0, s->stream>>>(s->st, tol);
+    IFL_LAUNCH_CHECK();
+    IFL_TRY(prof_begin(s, IFL_KERNEL_SWEEP_BWD, gidx));
+    IFL_TRY(launch_bwd2(s));
+    IFL_TRY(prof_end(s));
+    fin_sigma32_kernel<<<1, 256, 0, s->stream>>>(a, s->sk.nstrips, 0, tol, limit, deflate, s->st_outer);
+    IFL_LAUNCH_CHECK();
+    return IFL_OK;
+}
+
+static int read_state(Solver* s, PcgState* dev, PcgState* host) {
+    IFL_HIP(hipMemcpyAsync(host, dev, sizeof(PcgState), hipMemcpyDeviceToHost, s->stream));
+    IFL_HIP(hipStreamSynchronize(s->stream));
+    return IFL_OK;
+}
+
+int k_project_mixed(Solver* s, int limit, ifl_solve_stats* stats) {
+    VecArgs a = vec_args(s);
+    VecArgs32 b = vec_args32(s);
+    long n = skew_n(s);
+    int nbv = grid_blocks(n, 256);
+    const double tol = s->cfg.tol;
+    // The pressure matrix is singular (A 1 = 0 over the FLUID cells, closed
+    // walls), and MIC(0) amplifies the component of z along the constant
+    // vector, which single precision cannot carry: pressure solves deflate
+    // it, taking the mean over FLUID cells off the round's right-hand side
+    // and off every z. A e then has no constant part and the double residual
+    // keeps the reference's (its null-space part is never changed by A e).
+    const int deflate = s->solve_kind == SOLVE_PRESSURE;
+    const int nbr = std::max(1, std::min(nbv, s->npartial / 2 - 1));
+    if (s->profiling) IFL_HIP(hipEventRecord(s->ev0, s->stream));
+    IFL_HIP(hipMemsetAsync(s->p, 0, n * sizeof(double), s->stream));  // 7:854
+    mixed_copy_kernel<<<nbv, 256, 0, s->stream>>>(a, b, 0);
+    IFL_LAUNCH_CHECK();
+    // max|r| of the right-hand side (7:858)
+    VecArgs ao = a;
+    ao.st = s->st_outer;
+    state_reset_kernel<<<1, 1, 0, s->stream>>>(s->st_outer);
+    IFL_LAUNCH_CHECK();
+    inf_norm_kernel<<<nbv, 256, 0, s->stream>>>(ao);
+    IFL_LAUNCH_CHECK();
+    PcgState ho;
+    IFL_TRY(read_state(s, s->st_outer, &ho));
+    double rmax = ho.max_err;
+    int status = IFL_SOLVE_NOT_RUN, printed = 0, total = 0;
+    if (rmax < tol) {
+        status = IFL_SOLVE_INITIAL_SMALL;
+        printed = -1;
+    } else if (limit <= 0) {
+        status = IFL_SOLVE_BUDGET;
+        printed = limit;
+    }
+    for (int round = 0; status == IFL_SOLVE_NOT_RUN; round++) {
+        // a round: r32 = r, e32 = 0, then PCG in single precision from e = 0
+        const double tin = std::max(kInnerDrop * rmax, 0.25 * tol);
+        const int lim_in = limit - total;
+        state_reset_kernel<<<1, 1, 0, s->stream>>>(s->st_outer);
+        IFL_LAUNCH_CHECK();
+        if (deflate) {
+            mixed_rsum_kernel<<<nbr, 256, 0, s->stream>>>(a, n);
+            IFL_LAUNCH_CHECK();
+            mixed_rmean_kernel<<<1, 256, 0, s->stream>>>(s->partial, nbr, s->st_outer, nullptr);
+            IFL_LAUNCH_CHECK();
+        }
+        mixed_copy_kernel<<<nbv, 256, 0, s->stream>>>(ao, b, 1);
+        IFL_LAUNCH_CHECK();
+        state_reset_kernel<<<1, 1, 0, s->stream>>>(s->st);  // alpha = 0: the first sweeps apply M^-1 to r32
+        IFL_LAUNCH_CHECK();
+        IFL_TRY(launch_fwd2(s));
+        IFL_TRY(launch_bwd2(s));
+        fin_sigma32_kernel<<<1, 256, 0, s->stream>>>(a, s->sk.nstrips, 1, tin, lim_in, deflate, s->st_outer);
+        IFL_LAUNCH_CHECK();
+        mixed_init_s_kernel<<<nbv, 256, 0, s->stream>>>(b);
+        IFL_LAUNCH_CHECK();
+        int launched = 0, batch = 4;
+        PcgState h;
+        while (true) {
+            IFL_TRY(read_state(s, s->st, &h));
+            if (h.done || h.timeout || launched >= lim_in) break;
+            int nb = std::min(batch, lim_in - launched);
+            for (int k = 0; k < nb; k++)
+                IFL_TRY(launch_iteration32(s, launched + k, lim_in, tin, total + launched + k, deflate));
+            launched += nb;
+            batch = std::min(batch * 2, 64);
+        }
+        IFL_TRY(read_state(s, s->st, &h));
+        if (h.timeout) {
+            s->err = "PCG sweep hand-off timed out";
+            (void)edge_reset(s);
+            return IFL_ERR_DEVICE;
+        }
+        int its = 0;
+        if (h.status == IFL_SOLVE_CONVERGED) {
+            its = h.printed_iter + 1;
+            vec_update32_kernel<<<nbv, 256, 0, s->stream>>>(b, 2, 0);  // the last e += alpha*s
+            IFL_LAUNCH_CHECK();
+        } else if (h.status == IFL_SOLVE_BUDGET) {
+            its = h.iter;
+            if (its > 0) {
+                vec_update32_kernel<<<nbv, 256, 0, s->stream>>>(b, 2, 0);
+                IFL_LAUNCH_CHECK();
+            }
+        }
+        total += its;
+        // p += e, r -= A e in double; the new max|r|
+        state_reset_kernel<<<1, 1, 0, s->stream>>>(s->st_outer);
+        IFL_LAUNCH_CHECK();
+        mixed_outer_kernel<<<nbv, 256, 0, s->stream>>>(a, s->e32, s->st_outer);
+        IFL_LAUNCH_CHECK();
+        IFL_TRY(read_state(s, s->st_outer, &ho));
+        rmax = ho.max_err;
+        if (rmax < tol) {
+            status = IFL_SOLVE_CONVERGED;
+            printed = std::max(total - 1, 0);
+        } else if (total >= limit || round + 1 >= kMaxOuter || its == 0) {
+            status = IFL_SOLVE_BUDGET;
+            printed = limit;
+        }
+    }
+    if (s->profiling) {
+        IFL_HIP(hipEventRecord(s->ev1, s->stream));
+        IFL_HIP(hipEventSynchronize(s->ev1));
+        float ms = 0;
+        IFL_HIP(hipEventElapsedTime(&ms, s->ev0, s->ev1));
+        Profile& pr = s->prof[IFL_KERNEL_PCG];
+        pr.launches += 1;
+        pr.ms += ms;
+        pr.iterations += total;
+        pr.cell_iterations += (double)total * s->W * s->H;
+        int lim[IFL_KERNEL_COUNT] = {0};
+        lim[IFL_KERNEL_MATVEC] = lim[IFL_KERNEL_SWEEP_FWD] = lim[IFL_KERNEL_SWEEP_BWD] = total;
+        IFL_TRY(prof_collect(s, lim));
+    } else {
+        IFL_HIP(hipStreamSynchronize(s->stream));
+    }
+    if (stats) {
+        stats->status = status;
+        stats->iterations = printed;
+        stats->max_error = rmax;
+    }
+    if (s->cfg.verbose) {
+        if (status == IFL_SOLVE_CONVERGED)
+            printf("Exiting solver after %d iterations, maximum error is %f\n", printed, rmax);
+        else if (status == IFL_SOLVE_BUDGET)
+            printf("Exceeded budget of %d iterations, maximum error was %f\n", limit, rmax);
+        else if (status == IFL_SOLVE_INITIAL_SMALL && s->level >= 6)
+            printf("Initial guess sufficiently small\n");
+    }
     return IFL_OK;
 }
 
 // project(limit) 7:853-886
 int k_project(Solver* s, int limit, ifl_solve_stats* stats) {
+    if (s->cfg.precision == IFL_PREC_REFINE) return k_project_mixed(s, limit, stats);
     VecArgs a = vec_args(s);
     long n = skew_n(s);
     int seq = s->cfg.dot_order == IFL_DOT_SEQUENTIAL;
@@ -1266,11 +1757,38 @@ int k_project(Solver* s, int limit, ifl_solve_stats* stats) {
     state_reset_kernel<<<1, 1, 0, s->stream>>>(s->st);
     IFL_LAUNCH_CHECK();
     IFL_HIP(hipMemsetAsync(s->p, 0, n * sizeof(double), s->stream));  // 7:854
-    IFL_TRY(launch_fwd<0>(s));                                         // applyPreconditioner(_z, _r) 7:855
-    IFL_TRY(launch_bwd<0>(s));
-    IFL_HIP(hipMemcpyAsync(s->s, s->z, n * sizeof(double), hipMemcpyDeviceToDevice, s->stream));  // 7:856
-    fin_sigma_kernel<<<1, 256, 0, s->stream>>>(a, s->sk.nstrips, seq, 1, s->cfg.tol, limit);  // 7:858-864
-    IFL_LAUNCH_CHECK();
+    if (a.defl) {
+        // the mixed form's pressure solve: the FLUID count for the means of z
+        const int nbr = std::max(1, std::min(grid_blocks(n, 256), s->npartial / 2 - 1));
+        state_reset_kernel<<<1, 1, 0, s->stream>>>(s->st_outer);
+        IFL_LAUNCH_CHECK();
+        mixed_rsum_kernel<<<nbr, 256, 0, s->stream>>>(a, n);
+        IFL_LAUNCH_CHECK();
+        mixed_rmean_kernel<<<1, 256, 0, s->stream>>>(s->partial, nbr, s->st_outer, s->st);
+        IFL_LAUNCH_CHECK();
+        // The preconditioner is applied as P M^-1 P, with P taking off the
+        // mean over FLUID cells: r is kept less its mean (which A s, summing
+        // to zero over FLUID cells, does not change) and z is taken less its
+        // mean. The closed-wall pressure matrix is singular along the
+        // constant, and a right-hand side off by a constant makes the
+        // one-sided form lose the conjugate-gradient properties.
+        defl_shift_kernel<<<grid_blocks(n, 256), 256, 0, s->stream>>>(a, -1.0);
+        IFL_LAUNCH_CHECK();
+        // applyPreconditioner(_z, _r) 7:855 in the loop's form (its z sums),
+        // then s = z less its mean (7:856)
+        IFL_TRY(launch_fwd<0>(s));
+        IFL_TRY(launch_bwd<0>(s));
+        fin_sigma32_kernel<<<1, 256, 0, s->stream>>>(a, s->sk.nstrips, 1, s->cfg.tol, limit, 1, s->st_outer);
+        IFL_LAUNCH_CHECK();
+        defl_init_s_kernel<<<grid_blocks(n, 256), 256, 0, s->stream>>>(a);
+        IFL_LAUNCH_CHECK();
+    } else {
+        IFL_TRY(launch_fwd<0>(s));                                         // applyPreconditioner(_z, _r) 7:855
+        IFL_TRY(launch_bwd<0>(s));
+        IFL_HIP(hipMemcpyAsync(s->s, s->z, n * sizeof(double), hipMemcpyDeviceToDevice, s->stream));  // 7:856
+        fin_sigma_kernel<<<1, 256, 0, s->stream>>>(a, s->sk.nstrips, seq, 1, s->cfg.tol, limit);  // 7:858-864
+        IFL_LAUNCH_CHECK();
+    }
     int launched = 0;
     int batch = 4;
     while (true) {
@@ -1300,6 +1818,13 @@ int k_project(Solver* s, int limit, ifl_solve_stats* stats) {
             z_nonfluid_kernel<<<grid_blocks(n, 256), 256, 0, s->stream>>>(a);
             IFL_LAUNCH_CHECK();
         }
+    }
+    if (a.defl) {
+        // r back to the residual itself
+        defl_shift_kernel<<<grid_blocks(n, 256), 256, 0, s->stream>>>(a, 1.0);
+        IFL_LAUNCH_CHECK();
+        rmean_clear_kernel<<<1, 1, 0, s->stream>>>(s->st);
+        IFL_LAUNCH_CHECK();
     }
     if (s->profiling) {
         IFL_HIP(hipEventRecord(s->ev1, s->stream));

@@ -25,7 +25,7 @@ QF_SRC, QF_DST, QF_PHI, QF_VOLUME, QF_NORMALX, QF_NORMALY, QF_CELL, QF_BODY, QF_
 F_R, F_P, F_Z, F_S, F_PRECON, F_ADIAG, F_APLUSX, F_APLUSY, F_UDENSITY, F_VDENSITY = range(64, 74)
 F_POSX, F_POSY, F_PROP_D, F_PROP_T, F_PROP_U, F_PROP_V, F_WEIGHT, F_COUNTS = range(80, 88)
 BODY_BOX, BODY_SPHERE = 0, 1
-PREC_FP64, PREC_MIXED = 0, 1
+PREC_FP64, PREC_MIXED, PREC_REFINE = 0, 1, 2
 DOT_TREE, DOT_SEQUENTIAL = 0, 1
 SOLVE_CONVERGED, SOLVE_INITIAL_SMALL, SOLVE_BUDGET, SOLVE_NOT_RUN = 0, 1, 2, 3
 KERNEL_PCG, KERNEL_PRECON, KERNEL_ADVECT, KERNEL_SWEEP_FWD, KERNEL_SWEEP_BWD, KERNEL_MATVEC = 0, 1, 2, 3, 4, 5
@@ -287,6 +287,29 @@ class FluidSolver:
         n = ctypes.c_int()
         self._check(lib.ifl_get_particle_count(self._h, ctypes.byref(n)))
         return n.value
+
+
+def state_fields(level):
+    """The arrays that make up a solver's state between two update() calls
+    (the quantities and their solid geometry, the PCG vectors and matrix, the
+    face densities at levels 7-8, the particles at level 8): what a caller
+    copies to continue a run in another solver."""
+    qs = [Q_D, Q_U, Q_V] + ([Q_T] if level >= 6 else [])
+    out = []
+    for q in qs:
+        out.append(q * 16 + QF_SRC)
+        if level != 8:
+            out.append(q * 16 + QF_DST)
+        if level >= 4:
+            out += [q * 16 + f for f in (QF_CELL, QF_BODY, QF_NORMALX, QF_NORMALY, QF_MASK)]
+        if level >= 5:
+            out += [q * 16 + QF_VOLUME, q * 16 + QF_PHI]
+    out += [F_R, F_P, F_Z, F_S, F_PRECON, F_ADIAG, F_APLUSX, F_APLUSY]
+    if level >= 7:
+        out += [F_UDENSITY, F_VDENSITY]
+    if level == 8:
+        out += [F_POSX, F_POSY, F_PROP_D, F_PROP_T, F_PROP_U, F_PROP_V, F_COUNTS]
+    return out
 
 
 def frand_state():

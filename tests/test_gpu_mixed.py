@@ -1,12 +1,18 @@
-"""The mixed-precision form: an FP64 conjugate-gradient iteration with the
-MIC(0) preconditioner applied in single precision.
+"""The reduced-precision forms. IFL_PREC_MIXED: the conjugate-gradient
+iteration in double with the MIC(0) preconditioner applied in single
+precision; pressure solves apply it as P M^-1 P, with P taking off the mean
+over FLUID cells (the closed-wall pressure matrix is singular along the
+constant). IFL_PREC_REFINE: defect correction in double around a
+conjugate-gradient iteration entirely in single precision. Both advect in
+single precision.
 
-Its contract is the reference's own convergence test: the pressure (and
+Their contract is the reference's own convergence test: the pressure (and
 heat) solve stops when max |r| < 1e-5 over the reference's norm domain
 (7:872), with r the double-precision residual. The tests recompute that
 residual independently, with the oracle's double-precision
 matrixVectorProduct on the returned solution, and check the fields after a
-step against the FP64 form.
+step frame by frame from the same state against the converged solution
+(solved to 1e-9), to 1e-5 relative per field.
 """
 import numpy as np
 import pytest
@@ -39,11 +45,15 @@ def warm_oracle(level, w, h, bodies, steps):
 # with the reference's sequential sums and ~280 with pairwise sums), and some
 # scenes converge only for particular summation orders (solver 6 at 64x64),
 # so the contract checked is the residual, not the count.
-@pytest.mark.parametrize("level,w,h,bodies,warm", [
-    (3, 96, 80, None, 3),
-    (7, 130, 70, MULTI, 2),
-])
-def test_mixed_pressure_solve_meets_tolerance(level, w, h, bodies, warm):
+SCENES = [(3, 96, 80, None, 3), (7, 130, 70, MULTI, 2)]
+
+
+def _solve_forms(level, w, h, bodies, warm, precisions):
+    """The pressure system after `warm` oracle steps, solved by the oracle
+    (the reference's FP64 solve) and by the product in each precision: per
+    precision (iterations, status, max|r| reported, true residual, velocity
+    deviation from the reference's after applyPressure), plus the reference's
+    iterations and true residual."""
     import ifl_fluids as F
     orc, sc = warm_oracle(level, w, h, bodies, warm)
     dt = sc["dt"]
@@ -69,10 +79,16 @@ def test_mixed_pressure_solve_meets_tolerance(level, w, h, bodies, warm):
         return np.abs(rhs - orc.field(ol.F_Z))[dom].max()
 
     def velocities(p):
+        # applyPressure then setBoundaryCondition, as update() runs them: the
+        # faces of solid cells take the bodies' velocity, so only pressure
+        # differences between FLUID cells remain (the pressure of these
+        # closed-wall systems is fixed only up to a constant per region)
         for f in uv:
             orc.set_field(f, state[f])
         orc.set_field(ol.F_P, p)
         orc.stage("APPLY_PRESSURE", 0, dt)
+        if level >= 4:
+            orc.stage("SET_BOUNDARY")
         return [orc.field(f) for f in uv]
 
     limit = 600 if level == 3 else 2000
@@ -81,8 +97,8 @@ def test_mixed_pressure_solve_meets_tolerance(level, w, h, bodies, warm):
     ref_res = true_residual(p_ref)
     ref_uv = velocities(p_ref)
     bmax = np.abs(rhs[dom]).max()
-    dev = {}
-    for prec in (F.PREC_FP64, F.PREC_MIXED):
+    out = {}
+    for prec in precisions:
         gpu = make_product(level, w, h, sc["params"], sc["bodies"], precision=prec)
         gpu._push_bodies()
         for f, v in state.items():
@@ -98,21 +114,43 @@ def test_mixed_pressure_solve_meets_tolerance(level, w, h, bodies, warm):
         print(f"level {level} {w}x{h} precision {prec}: {got:.0f} iterations (reference {want_iters:.0f}); "
               f"max|r| {st.max_error:.2e}; true residual {res:.2e} (reference's own {ref_res:.2e}, "
               f"max|b| {bmax:.2e}); velocity after applyPressure vs the reference's: {duv:.2e}", flush=True)
+        out[prec] = (got, st.status, st.max_error, res, duv)
+    orc.close()
+    return out, want_iters, ref_res
+
+
+# Scenes whose solve converges in either precision. The pressure systems are
+# nearly singular (closed walls, MIC(0) with tau = 0.97): iteration counts
+# move with rounding alone (the 130x70 solver-7 solve takes 169 iterations
+# with the reference's sequential sums and ~280 with pairwise sums), and some
+# scenes converge only for particular summation orders (solver 6 at 64x64),
+# so the contract checked is the residual, not the count.
+@pytest.mark.parametrize("level,w,h,bodies,warm", SCENES)
+def test_mixed_pressure_solve_meets_tolerance(level, w, h, bodies, warm):
+    import ifl_fluids as F
+    out, want_iters, ref_res = _solve_forms(level, w, h, bodies, warm, (F.PREC_FP64, F.PREC_MIXED))
+    for prec, (got, status, max_err, res, duv) in out.items():
         # the reference's convergence test, on the double-precision residual
-        assert st.status == F.SOLVE_CONVERGED and st.max_error < 1e-5
+        assert status == F.SOLVE_CONVERGED and max_err < 1e-5
         # no less accurate than the reference's own solution
         assert res <= max(2.0 * ref_res, 1e-5)
-        dev[prec] = duv
-    # The velocities differ from the reference's as much as the drop-in's do
-    # with tree-ordered dot products (both are within the tolerance of a
-    # nearly singular system; see above).
-    assert dev[F.PREC_MIXED] <= 10.0 * dev[F.PREC_FP64] + 1e-6
-    orc.close()
+    # the deflated preconditioner takes no more iterations than the reference
+    assert out[F.PREC_MIXED][0] <= want_iters + 2
+
+
+@pytest.mark.parametrize("level,w,h,bodies,warm", SCENES)
+def test_refine_pressure_solve_meets_tolerance(level, w, h, bodies, warm):
+    import ifl_fluids as F
+    out, want_iters, ref_res = _solve_forms(level, w, h, bodies, warm, (F.PREC_REFINE,))
+    got, status, max_err, res, duv = out[F.PREC_REFINE]
+    assert status == F.SOLVE_CONVERGED and max_err < 1e-5
+    assert res <= max(2.0 * ref_res, 1e-5)
 
 
 def test_mixed_update_steps_against_fp64():
     """Solver 7 stepped in both forms from the same start: every solve
-    converges, and the fields stay close to the FP64 form's."""
+    converges and the fields stay finite and close to the FP64 form's (the
+    per-frame accuracy is held to the FP64 form's own spread below)."""
     import ifl_fluids as F
     level, w, h = 7, 128, 96
     sc = dict(ol.REF_SCENES[level])
@@ -134,4 +172,79 @@ def test_mixed_update_steps_against_fp64():
         assert np.isfinite(b).all()
         rel = np.abs(a - b).max() / max(np.abs(a).max(), 1e-30)
         print(f"field {f}: max relative difference mixed vs FP64 after 4 steps {rel:.2e}", flush=True)
-        assert rel < 1e-3
+        assert rel < 0.25
+
+
+def _frame_errors(level, n, scene, frames, warm, prec=None):
+    """Teacher-forced frames: from the FP64 drop-in's state after `warm`
+    steps, one update() by the drop-in, by the mixed form and by the FP64
+    form solving to a tolerance of 1e-9 (the converged solution); per field,
+    the max relative difference of the drop-in and of the mixed form from the
+    converged step."""
+    import ifl_fluids as F
+    from gpu_util import copy_state
+    fields = [ol.qfield(q, ol.QF_SRC) for q in (ol.Q_D, ol.Q_T, ol.Q_U, ol.Q_V)] + [ol.F_P]
+    state_fields = ol.state_fields(level)
+    ref = make_product(level, n, n, scene["params"], scene["bodies"])
+    tight = make_product(level, n, n, scene["params"], scene["bodies"], tol=1e-9, pcg_limit=100000)
+    mix = make_product(level, n, n, scene["params"], scene["bodies"],
+                       precision=F.PREC_MIXED if prec is None else prec)
+    for _ in range(warm):
+        for inf in scene["inflows"]:
+            ref.add_inflow(*inf)
+        ref.update(scene["dt"])
+    out = []
+    for k in range(frames):
+        for other in (tight, mix):
+            copy_state(ref, other, state_fields)
+        for sv in (ref, tight, mix):
+            for inf in scene["inflows"]:
+                sv.add_inflow(*inf)
+        sts = [sv.update(scene["dt"]) for sv in (ref, tight, mix)]
+        for st in sts:
+            assert st.pressure.status == F.SOLVE_CONVERGED
+        errs = {}
+        fl = tight.field(ol.qfield(ol.Q_D, ol.QF_CELL)) == 0
+        for f in fields:
+            vals = [sv.field(f) for sv in (tight, ref, mix)]
+            if f == ol.F_P:
+                # pressure less its mean over FLUID cells: the FP64 iteration
+                # lets the constant drift, the mixed form deflates it
+                vals = [np.where(fl, v - v[fl].mean(), 0.0) for v in vals]
+            a = vals[0]
+            scale = max(np.abs(a).max(), 1e-30)
+            errs[f] = (np.abs(vals[1] - a).max() / scale, np.abs(vals[2] - a).max() / scale)
+        out.append((errs, [st.pressure.iterations for st in sts]))
+    for sv in (ref, tight, mix):
+        sv.close()
+    return out
+
+
+@pytest.mark.parametrize("n,warm", [(256, 6), (512, 2)])
+def test_mixed_frames_against_converged(n, warm):
+    """The benchmark scene (solver 7, two seeded hot inflows), teacher-forced
+    frame by frame: every field of the mixed form's step within 1e-5
+    relative of the converged step (tolerance 1e-9), as the drop-in's is."""
+    import ifl_scenes as S
+    sc = S.config3_scene()
+    for k, (errs, iters) in enumerate(_frame_errors(7, n, sc, 2, warm)):
+        for f, (e_ref, e_mix) in errs.items():
+            print(f"{n}x{n} frame {k} field {f}: drop-in {e_ref:.2e}, mixed {e_mix:.2e} from the converged step "
+                  f"(pressure iterations drop-in/converged/mixed {iters})", flush=True)
+        for f, (e_ref, e_mix) in errs.items():
+            assert e_ref < 1e-5
+            assert np.isfinite(e_mix) and e_mix < 1e-5
+
+
+def test_refine_frames_against_converged():
+    """The defect-correction form at 256x256, teacher-forced as above: within
+    1e-5 relative of the converged step per field."""
+    import ifl_fluids as F
+    import ifl_scenes as S
+    sc = S.config3_scene()
+    for k, (errs, iters) in enumerate(_frame_errors(7, 256, sc, 2, 6, prec=F.PREC_REFINE)):
+        for f, (e_ref, e_mix) in errs.items():
+            print(f"256x256 frame {k} field {f}: drop-in {e_ref:.2e}, refine {e_mix:.2e} from the converged step "
+                  f"(pressure iterations drop-in/converged/refine {iters})", flush=True)
+        for f, (e_ref, e_mix) in errs.items():
+            assert np.isfinite(e_mix) and e_mix < 1e-5
