@@ -33,6 +33,9 @@ KERNEL_BYTES_PER_CELL = {
     # adiag, aplusx, aplusy, s read; q written
     "matvec": 40.0,
 }
+# the same for solver 4 (the batched form): its backward sweep also reads
+# the reference's _z at non-FLUID cells (A s, kept in q) for the unmasked dot
+BATCH_BYTES_PER_CELL = dict(KERNEL_BYTES_PER_CELL, sweep_bwd=56.0)
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "traffic.json")
 
 
@@ -52,6 +55,7 @@ def parse():
     ap.add_argument("--form", choices=["fp64", "mixed", "refine"], default=None,
                     help="time only this form (default: FP64 as the value, the mixed form beside it)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-batch", action="store_true", help="solver7: skip the batched form's line")
     ap.add_argument("--cpu-iters", type=int, default=30)
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run a plain 1 GiB device copy (scale for rocprofv3 byte counters) and exit")
@@ -399,20 +403,22 @@ def _cpu_baseline_batch(insts, sample):
                 sample=f"first update() of the first {sample} instances ({cells} cells) on one host core: {secs:.1f} s")
 
 
-def run_batch(args):
+def measure_batch(n_inst, steps, warmup, mode="fused", threads=16):
     """Config 4: independent solver-4 instances (sides drawn from 64..512,
-    one static box and one inflow each, ifl_scenes.config4_instances), each
-    a solver handle with its own HIP stream, stepped by `--threads` host
-    threads (the library releases the GIL in every call). Metric: total
-    cells x steps / s."""
+    one static box and one inflow each, ifl_scenes.config4_instances),
+    stepped as one ifl_batch (mode "fused") or each on its own stream from
+    `threads` host threads (mode "streams"; the library releases the GIL in
+    every call). Returns the measurement: total cells x steps / s, and in the
+    fused mode the fused PCG launches' times (HIP events on the batch's
+    stream) with the cells that worked in them."""
     import threading
     import ifl_fluids as F
     import ifl_scenes as S
-    insts = S.config4_instances(args.instances)
+    insts = S.config4_instances(n_inst)
     solvers = [F.FluidSolver(4, i["w"], i["h"], i["params"], [F.SolidBox(*b[1:]) for b in i["bodies"]])
                for i in insts]
     cells = sum(i["w"] * i["h"] for i in insts)
-    nt = max(1, args.threads)
+    nt = max(1, threads)
     errors = []
 
     def run(idx, steps, iters):
@@ -438,7 +444,8 @@ def run_batch(args):
             t.join()
         return time.perf_counter() - t0, iters
 
-    if args.batch_mode == "fused":
+    batch = None
+    if mode == "fused":
         batch = F.Batch(solvers)
 
         def step_all(steps):  # noqa: F811 - the fused form replaces the threaded one
@@ -453,26 +460,64 @@ def run_batch(args):
                     iters[k] += x.pressure.iterations
             return time.perf_counter() - t0, iters
 
-    step_all(args.warmup)
-    wall, iters = step_all(args.steps)
-    if args.batch_mode == "fused":
+    step_all(warmup)
+    wall, iters = step_all(steps)
+    kern = {}
+    if batch is not None:
+        # a separate profiled step: events around every fused launch
+        batch.profile(True)
+        step_all(1)
+        for name, kid in (("sweep_fwd", F.KERNEL_SWEEP_FWD), ("sweep_bwd", F.KERNEL_SWEEP_BWD),
+                          ("matvec", F.KERNEL_MATVEC)):
+            p = batch.profile_read(kid)
+            if p["launches"] == 0 or p["ms"] <= 0:
+                continue
+            byts = BATCH_BYTES_PER_CELL[name] * p["cell_iterations"]
+            kern[name] = {"launches": p["launches"], "total_ms": p["ms"], "avg_us": 1e3 * p["ms"] / p["launches"],
+                          "cell_iterations": p["cell_iterations"], "bytes": byts,
+                          "achieved_GBps": byts / (p["ms"] * 1e-3) / 1e9}
+        batch.profile(False)
         batch.close()
     for sv in solvers:
         sv.close()
     if errors:
-        print(json.dumps({"error": errors[:3]}))
+        raise RuntimeError("; ".join(errors[:3]))
+    out = {"value": cells * steps / wall, "unit": "cell-updates/s", "steps": steps, "warmup": warmup,
+           "ms_per_step": 1e3 * wall / steps, "instances": len(insts), "cells": cells, "mode": mode,
+           "threads": nt if mode == "streams" else 1, "streams": len(insts) if mode == "streams" else 1,
+           "pressure_iterations_per_step": sum(iters) / steps, "insts": insts}
+    if kern:
+        out["kernels"] = kern
+        dom = max(kern, key=lambda k: kern[k]["total_ms"])
+        k = kern[dom]
+        out["roofline"] = {"bound": "hbm", "achieved": k["achieved_GBps"], "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                           "frac": k["achieved_GBps"] / PEAK_HBM_GBS, "traffic": None,
+                           "kernel": f"fused {dom} over all instances: {BATCH_BYTES_PER_CELL[dom]:.0f} B/cell x the "
+                                     f"cells of the instances iterating in each launch, over the launches' summed "
+                                     f"time (HIP events on the batch's stream, one profiled step)"}
+    return out
+
+
+def run_batch(args):
+    """`--workload batch`: config 4 as the headline (cell-updates/s)."""
+    try:
+        m = measure_batch(args.instances, args.steps, args.warmup, args.batch_mode, args.threads)
+    except RuntimeError as e:
+        print(json.dumps({"error": str(e)}))
         sys.exit(1)
+    insts = m.pop("insts")
     out = {
         "metric": "cell-updates/s, 256 independent solver-4 instances of mixed sizes, FP64 drop-in",
-        "value": cells * args.steps / wall, "unit": "cell-updates/s", "n_gpus": 1, "steps": args.steps,
-        "warmup": args.warmup, "ms_per_step": 1e3 * wall / args.steps, "higher_is_better": True, "scaling": "weak",
+        "value": m["value"], "unit": "cell-updates/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": m["ms_per_step"], "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "f64", "data": "synthetic",
-        "config": {"workload": f"solver4-batch-{len(insts)}-instances-64to512", "instances": len(insts),
-                   "cells": cells, "mode": args.batch_mode,
-                   "threads": nt if args.batch_mode == "streams" else 1,
-                   "streams": len(insts) if args.batch_mode == "streams" else 1,
-                   "pressure_iterations_per_step": sum(iters) / args.steps},
+        "config": {"workload": f"solver4-batch-{m['instances']}-instances-64to512", "instances": m["instances"],
+                   "cells": m["cells"], "mode": m["mode"], "threads": m["threads"], "streams": m["streams"],
+                   "pressure_iterations_per_step": m["pressure_iterations_per_step"]},
     }
+    if "kernels" in m:
+        out["kernels"] = m["kernels"]
+        out["roofline"] = m["roofline"]
     if not args.no_cpu_baseline:
         try:
             out["cpu_baseline"] = cpu_baseline_batch(insts, 16)
@@ -510,7 +555,9 @@ def main():
     # --form the mixed form is run too and reported beside it
     main_form = args.form or "fp64"
     forms = [main_form] if args.form else ["fp64", "mixed", "refine"]
-    res = {f: run_form(f, n, args.steps, args.warmup) for f in forms}
+    # the forms beside the headline run at most 4 steps after 1 (bounded run time)
+    res = {f: run_form(f, n, args.steps, args.warmup) if f == main_form
+           else run_form(f, n, min(args.steps, 4), min(args.warmup, 1)) for f in forms}
     r = res[main_form]
     kern = r["kernels"]
     dom = max(kern, key=lambda k: kern[k]["total_ms"]) if kern else None
@@ -542,6 +589,7 @@ def main():
                                      f"average over {k['launches']} launches (HIP events on the solver's stream)"}
     if len(forms) > 1:
         out["forms"] = {f: {"steps_per_s": res[f]["steps_per_s"], "ms_per_step": res[f]["ms_per_step"],
+                            "steps": len(res[f]["solves"]),
                             "pcg_iterations_per_step": res[f]["mean_iters"],
                             "ms_per_pcg_iteration": res[f]["pcg"]["ms_per_iteration"],
                             "sweep_us": {k: v["avg_us"] for k, v in res[f]["kernels"].items()}} for f in forms}
@@ -551,6 +599,15 @@ def main():
                 out["forms"][f"{f}_frame_error"] = mixed_frame_errors(n, form=f)
             except Exception as e:  # reported beside the measurement, never in place of it
                 out["forms"][f"{f}_frame_error"] = {"error": str(e)}
+    if not args.no_batch:
+        # config 4 beside the headline: the batched form (one fused launch per
+        # PCG phase for all instances), with its own roofline
+        try:
+            m = measure_batch(256, 2, 1)
+            m.pop("insts")
+            out["batch"] = m
+        except Exception as e:  # reported beside the measurement, never in place of it
+            out["batch"] = {"error": str(e)}
     if not args.no_cpu_baseline:
         try:
             out["cpu_baseline"] = cpu_baseline(r["scene"], n, r["mean_iters"])

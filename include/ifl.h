@@ -214,6 +214,15 @@ typedef struct ifl_batch ifl_batch;
 int ifl_batch_create(ifl_solver* const* solvers, int n, ifl_batch** out);
 int ifl_batch_update(ifl_batch* batch, double dt, ifl_step_stats* stats);
 int ifl_batch_destroy(ifl_batch* batch);
+/* Diagnostics, no counterpart in the reference: with profiling on, every
+ * fused PCG launch of the batch (IFL_KERNEL_MATVEC, _SWEEP_FWD, _SWEEP_BWD)
+ * is timed with events on the batch's stream. ifl_batch_profile(on) also
+ * clears the totals; ifl_batch_profile_read returns, per kernel, the timed
+ * launches, their total milliseconds and the cells of the instances that
+ * worked in them, summed over the launches. */
+int ifl_batch_profile(ifl_batch* batch, int on);
+int ifl_batch_profile_read(const ifl_batch* batch, int kernel, int64_t* launches, double* ms,
+                           double* cell_iterations);
 
 /* FluidSolver::toImage(rgba[, renderHeat]) 3:455 / 4:903 / 5:961 / 6:1021 /
  * 7:1047 / 8:1426, rendered on the device: bytes = w*h*4, or w*h*8 with

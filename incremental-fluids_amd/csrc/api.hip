@@ -22,6 +22,8 @@ struct ifl_batch {
     hipStream_t stream = nullptr;
     int device = 0;
     ifl::BatchDev* dev = nullptr;
+    bool profiling = false;
+    ifl::BatchProfile prof;
 };
 
 namespace ifl {
@@ -942,7 +944,7 @@ int ifl_batch_update(ifl_batch* b, double dt, ifl_step_stats* stats) {
         }
         if (rc != IFL_OK || next == SOLVE_NONE) break;
         rc = k_project_batch(b->solvers.data(), n, b->stream, &b->dev,
-                             next == SOLVE_HEAT ? heat.data() : press.data());
+                             next == SOLVE_HEAT ? heat.data() : press.data(), b->profiling ? &b->prof : nullptr);
         if (rc != IFL_OK) g_err = b->solvers[0]->err;
         after = next;
     }
@@ -952,6 +954,21 @@ int ifl_batch_update(ifl_batch* b, double dt, ifl_step_stats* stats) {
     }
     (void)hipStreamSynchronize(b->stream);
     return rc;
+}
+
+int ifl_batch_profile(ifl_batch* b, int on) {
+    if (!b) return IFL_ERR_ARG;
+    b->profiling = on != 0;
+    b->prof = ifl::BatchProfile();
+    return IFL_OK;
+}
+
+int ifl_batch_profile_read(const ifl_batch* b, int kernel, int64_t* launches, double* ms, double* cell_iterations) {
+    if (!b || kernel < 0 || kernel >= IFL_KERNEL_COUNT) return IFL_ERR_ARG;
+    if (launches) *launches = b->prof.launches[kernel];
+    if (ms) *ms = b->prof.ms[kernel];
+    if (cell_iterations) *cell_iterations = b->prof.cell_iterations[kernel];
+    return IFL_OK;
 }
 
 int ifl_batch_destroy(ifl_batch* b) {

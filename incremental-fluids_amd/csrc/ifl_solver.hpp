@@ -207,7 +207,16 @@ int k_project(Solver* s, int limit, ifl_solve_stats* stats);
 // tail on its own stream, the iterations on `stream`, one launch per phase of
 // each PCG iteration for all of them (stats: n entries)
 struct BatchDev;
-int k_project_batch(Solver* const* ss, int n, hipStream_t stream, BatchDev** dev, ifl_solve_stats* stats);
+// per-kernel totals of a batch's fused PCG launches (when profiling): every
+// launch timed with events on the batch's stream; cell_iterations counts the
+// cells of the instances that did work in them
+struct BatchProfile {
+    long long launches[IFL_KERNEL_COUNT] = {};
+    double ms[IFL_KERNEL_COUNT] = {};
+    double cell_iterations[IFL_KERNEL_COUNT] = {};
+};
+int k_project_batch(Solver* const* ss, int n, hipStream_t stream, BatchDev** dev, ifl_solve_stats* stats,
+                    BatchProfile* prof);
 void batch_dev_free(BatchDev* d);
 // per-launch timing (when profiling): events on the solver's stream around
 // a launch; prof_collect() adds the elapsed times of launches whose index is

@@ -1099,11 +1099,11 @@ static StreamArgs stream_args(Solver* s, int kind) {
 }
 
 template <int KIND, int MODE, int LV, int NG, typename T>
-static int ring_setup(Solver* s, StreamArgs& a, size_t* shmem) {
+static int ring_setup(Solver* s, StreamArgs& a, size_t* shmem, int max_nb = 12) {
     a.slot_bytes = SweepShape<KIND, MODE, NG, T>::SLOT;
     size_t fixed = sizeof(RingHdr) + kERing * sizeof(double);
     int nb = (int)((s->lds_bytes - fixed) / a.slot_bytes);
-    if (nb > 12) nb = 12;
+    if (nb > max_nb) nb = max_nb;
     if (nb < 3) {
         s->err = "not enough LDS for the sweep ring";
         return IFL_ERR_UNSUPPORTED;
@@ -1865,12 +1865,13 @@ int k_project(Solver* s, int limit, ifl_solve_stats* stats) {
 // ---------------------------------------------------------------- batched solves
 // project() of several independent solvers of one level together: each phase
 // of a PCG iteration is one launch for all of them, its blocks dealt from a
-// table of (solver, block) pairs. Sweep strips are listed upstream first
-// inside each solver (ascending for the forward sweep, descending for the
+// table of (solver, block) pairs. Sweep strips are listed by rank from the
+// sweep's start (ascending strips for the forward sweep, descending for the
 // backward one), so a strip's upstream neighbour always has the lower block
 // id: it is dispatched first, and the strips need not all be resident at
 // once however many solvers the batch holds. Solvers that have converged
-// return at once from every launch.
+// leave the tables at the host's next read of the convergence flags, and
+// return at once from every launch until then.
 struct BatchInst {
     VecArgs v;
     int nbv;      // vector blocks of 256 cells
@@ -1895,6 +1896,15 @@ struct BatchDev {
     std::vector<Solver*> solvers; // the list the tables were built for
     std::vector<hipEvent_t> ready;  // per solver: its entry into the solve is queued
     hipEvent_t joined = nullptr;    // the batched iterations are queued
+    // the full block tables; the device tables hold those of the instances
+    // still iterating (rebuilt when some have finished)
+    std::vector<int2> vmap_h, fmap_h, bmap_h, vmap_c, fmap_c, bmap_c;
+    int nvblk_full = 0, nsblk_full = 0;
+    bool compacted = false;
+    // profiling: events around the fused launches (matvec, forward and
+    // backward sweep) of the current solve
+    std::vector<hipEvent_t> ev_pool;
+    std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> timed;
 };
 
 void batch_dev_free(BatchDev* d) {
@@ -1903,6 +1913,11 @@ void batch_dev_free(BatchDev* d) {
     F(d->inst); F(d->fwd); F(d->bwd); F(d->vmap); F(d->fmap); F(d->bmap); F(d->states); F(d->st_ptrs);
     if (d->states_host) (void)hipHostFree(d->states_host);
     for (hipEvent_t e : d->ready) (void)hipEventDestroy(e);
+    for (hipEvent_t e : d->ev_pool) (void)hipEventDestroy(e);
+    for (auto& t : d->timed) {
+        (void)hipEventDestroy(t.second.first);
+        (void)hipEventDestroy(t.second.second);
+    }
     if (d->joined) (void)hipEventDestroy(d->joined);
     delete d;
 }
@@ -1989,14 +2004,39 @@ int batch_build(Solver* const* ss, int n, BatchDev* d) {
         d->shmem_f = shf;
         d->shmem_b = shb;
         for (int k = 0; k < b.nbv; k++) vmap.push_back(make_int2(i, k));
-        for (int j = 0; j < b.nstrips; j++) fmap.push_back(make_int2(i, j));
-        for (int j = b.nstrips - 1; j >= 0; j--) bmap.push_back(make_int2(i, j));
         sts[i] = s->st;
+    }
+    // Strips are dealt by their distance from the sweep's start (every
+    // instance's first strip, then every second strip, ...), larger
+    // instances first within a rank. A strip's upstream neighbour thus has a
+    // lower block id, so the strips need not be resident together; and a
+    // strip is dispatched after the previous rank, so it seldom holds a CU
+    // while it waits for its upstream neighbour (instance-major order, all
+    // strips of an instance together: 230 / 206 us per forward / backward
+    // launch on the 256-instance draw; this order: 184 / 173 us).
+    {
+        std::vector<int> order(n);
+        for (int i = 0; i < n; i++) order[i] = i;
+        std::stable_sort(order.begin(), order.end(), [&](int x, int y) {
+            return (long)ss[x]->W * ss[x]->H > (long)ss[y]->W * ss[y]->H;
+        });
+        int most = 0;
+        for (int i = 0; i < n; i++) most = std::max(most, ss[i]->sk.nstrips);
+        for (int r = 0; r < most; r++)
+            for (int i : order) {
+                const int ns = ss[i]->sk.nstrips;
+                if (r >= ns) continue;
+                fmap.push_back(make_int2(i, r));
+                bmap.push_back(make_int2(i, ns - 1 - r));
+            }
     }
     Solver* s = s0;  // error reporting for the uploads
     d->n = n;
-    d->nvblk = (int)vmap.size();
-    d->nsblk = (int)fmap.size();
+    d->nvblk = d->nvblk_full = (int)vmap.size();
+    d->nsblk = d->nsblk_full = (int)fmap.size();
+    d->vmap_h = vmap;
+    d->fmap_h = fmap;
+    d->bmap_h = bmap;
     IFL_TRY(upload(s, &d->inst, inst));
     IFL_TRY(upload(s, &d->fwd, fwd));
     IFL_TRY(upload(s, &d->bwd, bwd));
@@ -2013,24 +2053,85 @@ int batch_build(Solver* const* ss, int n, BatchDev* d) {
     return batch_attrs<LV, NGF, NGB>(s, d);
 }
 
+// events around one fused launch when profiling
+int batch_time(Solver* s, BatchDev* d, hipStream_t stream, int kernel, bool begin) {
+    if (begin) {
+        hipEvent_t a;
+        if (d->ev_pool.empty()) {
+            IFL_HIP(hipEventCreate(&a));
+        } else {
+            a = d->ev_pool.back();
+            d->ev_pool.pop_back();
+        }
+        IFL_HIP(hipEventRecord(a, stream));
+        d->timed.push_back({kernel, {a, nullptr}});
+        return IFL_OK;
+    }
+    hipEvent_t b;
+    if (d->ev_pool.empty()) {
+        IFL_HIP(hipEventCreate(&b));
+    } else {
+        b = d->ev_pool.back();
+        d->ev_pool.pop_back();
+    }
+    IFL_HIP(hipEventRecord(b, stream));
+    d->timed.back().second.second = b;
+    return IFL_OK;
+}
+
 template <int LV, int NGF, int NGB>
-int batch_iteration(Solver* s, BatchDev* d, hipStream_t stream, int it) {
+int batch_iteration(Solver* s, BatchDev* d, hipStream_t stream, int it, bool prof) {
     if (it > 0) {
         batch_vec_update_kernel<<<d->nvblk, 256, 0, stream>>>(d->inst, d->vmap);
         IFL_LAUNCH_CHECK();
     }
+    if (prof) IFL_TRY(batch_time(s, d, stream, IFL_KERNEL_MATVEC, true));
     batch_matvec_kernel<<<d->nvblk, 256, 0, stream>>>(d->inst, d->vmap);
     IFL_LAUNCH_CHECK();
+    if (prof) IFL_TRY(batch_time(s, d, stream, IFL_KERNEL_MATVEC, false));
     batch_fin_alpha_kernel<<<d->n, 256, 0, stream>>>(d->inst);
     IFL_LAUNCH_CHECK();
+    if (prof) IFL_TRY(batch_time(s, d, stream, IFL_KERNEL_SWEEP_FWD, true));
     batch_sweep_kernel<SW_FWD, LV, NGF><<<d->nsblk, 64 * kWaves, d->shmem_f, stream>>>(d->fwd, d->fmap);
     IFL_LAUNCH_CHECK();
+    if (prof) IFL_TRY(batch_time(s, d, stream, IFL_KERNEL_SWEEP_FWD, false));
     batch_fin_check_kernel<<<grid_blocks(d->n, 256), 256, 0, stream>>>(d->inst, d->n);
     IFL_LAUNCH_CHECK();
+    if (prof) IFL_TRY(batch_time(s, d, stream, IFL_KERNEL_SWEEP_BWD, true));
     batch_sweep_kernel<SW_BWD, LV, NGB><<<d->nsblk, 64 * kWaves, d->shmem_b, stream>>>(d->bwd, d->bmap);
     IFL_LAUNCH_CHECK();
+    if (prof) IFL_TRY(batch_time(s, d, stream, IFL_KERNEL_SWEEP_BWD, false));
     batch_fin_sigma_kernel<<<d->n, 256, 0, stream>>>(d->inst);
     IFL_LAUNCH_CHECK();
+    return IFL_OK;
+}
+
+// The block tables of the instances in `active` (all when null). The host
+// copies stay in BatchDev until the next rebuild; every launch that reads the
+// device tables has completed (the caller has synchronised the stream).
+int batch_tables(Solver* s, BatchDev* d, hipStream_t stream, const std::vector<char>* active) {
+    if (!active) {
+        if (!d->compacted) return IFL_OK;
+        d->vmap_c = d->vmap_h;
+        d->fmap_c = d->fmap_h;
+        d->bmap_c = d->bmap_h;
+    } else {
+        auto pick = [&](const std::vector<int2>& full, std::vector<int2>& out) {
+            out.clear();
+            for (const int2& m : full)
+                if ((*active)[m.x]) out.push_back(m);
+        };
+        pick(d->vmap_h, d->vmap_c);
+        pick(d->fmap_h, d->fmap_c);
+        pick(d->bmap_h, d->bmap_c);
+    }
+    d->nvblk = (int)d->vmap_c.size();
+    d->nsblk = (int)d->fmap_c.size();
+    IFL_HIP(hipMemcpyAsync(d->vmap, d->vmap_c.data(), d->vmap_c.size() * sizeof(int2), hipMemcpyHostToDevice, stream));
+    IFL_HIP(hipMemcpyAsync(d->fmap, d->fmap_c.data(), d->fmap_c.size() * sizeof(int2), hipMemcpyHostToDevice, stream));
+    IFL_HIP(hipMemcpyAsync(d->bmap, d->bmap_c.data(), d->bmap_c.size() * sizeof(int2), hipMemcpyHostToDevice, stream));
+    IFL_HIP(hipStreamSynchronize(stream));  // the host tables are rebuilt in place next time
+    d->compacted = active != nullptr;
     return IFL_OK;
 }
 
@@ -2046,7 +2147,8 @@ int batch_gather(Solver* s, BatchDev* d, hipStream_t stream) {
 
 // The batched project(): the entry (7:854-864) and the exit tails per solver,
 // exactly as k_project does them, and the iterations in between batched.
-int k_project_batch(Solver* const* ss, int n, hipStream_t stream, BatchDev** dev, ifl_solve_stats* stats) {
+int k_project_batch(Solver* const* ss, int n, hipStream_t stream, BatchDev** dev, ifl_solve_stats* stats,
+                    BatchProfile* prof) {
     Solver* s0 = ss[0];
     const int lv = s0->level;
     if (lv == 8 || s0->cfg.precision != IFL_PREC_FP64 || s0->cfg.dot_order != IFL_DOT_TREE) {
@@ -2083,10 +2185,12 @@ int k_project_batch(Solver* const* ss, int n, hipStream_t stream, BatchDev** dev
         limit = std::max(limit, s->cfg.pcg_limit);
     }
     Solver* s = s0;
-    int launched = 0, batch = 4;
+    IFL_TRY(batch_tables(s, d, stream, nullptr));  // every instance iterates at first
+    int launched = 0, batch = 4, iterating = n;
+    std::vector<char> active(n, 1);
     while (true) {
         IFL_TRY(batch_gather(s, d, stream));
-        bool all = true;
+        int still = 0;
         for (int i = 0; i < n; i++) {
             if (d->states_host[i].timeout) {
                 ss[i]->err = "PCG sweep hand-off timed out";
@@ -2094,18 +2198,50 @@ int k_project_batch(Solver* const* ss, int n, hipStream_t stream, BatchDev** dev
                 s0->err = ss[i]->err;
                 return IFL_ERR_DEVICE;
             }
-            all = all && d->states_host[i].done;
+            active[i] = !d->states_host[i].done;
+            still += active[i];
         }
-        if (all || launched >= limit) break;
+        if (still == 0 || launched >= limit) break;
+        // the finished instances' blocks leave the tables (they would return
+        // at once, but each still takes a dispatch slot and, in the sweeps, a
+        // whole CU's worth of LDS while it is resident)
+        if (still < iterating) {
+            IFL_TRY(batch_tables(s, d, stream, &active));
+            iterating = still;
+        }
         const int nb = std::min(batch, limit - launched);
         for (int k = 0; k < nb; k++) {
-            int rc = lv == 3 ? batch_iteration<3, 5, 5>(s, d, stream, launched + k)
-                   : lv <= 5 ? batch_iteration<4, 5, 6>(s, d, stream, launched + k)
-                             : batch_iteration<6, 5, 5>(s, d, stream, launched + k);
+            int rc = lv == 3 ? batch_iteration<3, 5, 5>(s, d, stream, launched + k, prof != nullptr)
+                   : lv <= 5 ? batch_iteration<4, 5, 6>(s, d, stream, launched + k, prof != nullptr)
+                             : batch_iteration<6, 5, 5>(s, d, stream, launched + k, prof != nullptr);
             if (rc != IFL_OK) return rc;
         }
         launched += nb;
         batch = std::min(batch * 2, 64);
+    }
+    if (prof) {
+        IFL_HIP(hipStreamSynchronize(stream));
+        for (auto& t : d->timed) {
+            float ms = 0;
+            IFL_HIP(hipEventElapsedTime(&ms, t.second.first, t.second.second));
+            prof->launches[t.first] += 1;
+            prof->ms[t.first] += ms;
+            d->ev_pool.push_back(t.second.first);
+            d->ev_pool.push_back(t.second.second);
+        }
+        d->timed.clear();
+        // the cells of the instances that worked in each fused launch: an
+        // instance that converged in iteration m ran m + 1 products and
+        // forward sweeps and m backward sweeps; one out of budget ran all
+        for (int i = 0; i < n; i++) {
+            const PcgState& h = d->states_host[i];
+            const double cells = (double)ss[i]->W * ss[i]->H;
+            const int f = h.status == IFL_SOLVE_CONVERGED ? h.printed_iter + 1 : h.iter;
+            const int b = h.status == IFL_SOLVE_CONVERGED ? h.printed_iter : h.iter;
+            prof->cell_iterations[IFL_KERNEL_MATVEC] += cells * f;
+            prof->cell_iterations[IFL_KERNEL_SWEEP_FWD] += cells * f;
+            prof->cell_iterations[IFL_KERNEL_SWEEP_BWD] += cells * b;
+        }
     }
     // the tails back on the solvers' streams, after the batched iterations
     IFL_HIP(hipEventRecord(d->joined, stream));

@@ -104,6 +104,9 @@ def _load():
     lib.ifl_batch_create.argtypes = [ctypes.POINTER(vp), ctypes.c_int, ctypes.POINTER(vp)]
     lib.ifl_batch_update.argtypes = [vp, ctypes.c_double, ctypes.POINTER(StepStats)]
     lib.ifl_batch_destroy.argtypes = [vp]
+    lib.ifl_batch_profile.argtypes = [vp, ctypes.c_int]
+    lib.ifl_batch_profile_read.argtypes = [vp, ctypes.c_int, ctypes.POINTER(ctypes.c_int64),
+                                           ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double)]
     lib.ifl_write_png.argtypes = [ctypes.c_char_p, vp, ctypes.c_int, ctypes.c_int]
     lib.ifl_frand_state.restype = ctypes.c_uint32
     lib.ifl_frand_state.argtypes = []
@@ -361,6 +364,22 @@ class Batch:
         if rc != 0:
             raise IflError(f"ifl_batch_update failed rc={rc}: {lib.ifl_last_error(None).decode()}")
         return list(st)
+
+    def profile(self, on=True):
+        """Times every fused PCG launch (and clears the totals)."""
+        lib.ifl_batch_profile(self._h, 1 if on else 0)
+
+    def profile_read(self, kernel):
+        """Totals of one fused kernel (KERNEL_MATVEC, KERNEL_SWEEP_FWD,
+        KERNEL_SWEEP_BWD): launches, ms and the cells of the instances that
+        worked in them, summed over the launches."""
+        n = ctypes.c_int64()
+        ms = ctypes.c_double()
+        ci = ctypes.c_double()
+        rc = lib.ifl_batch_profile_read(self._h, kernel, ctypes.byref(n), ctypes.byref(ms), ctypes.byref(ci))
+        if rc != 0:
+            raise IflError(f"ifl_batch_profile_read failed rc={rc}")
+        return {"launches": n.value, "ms": ms.value, "cell_iterations": ci.value}
 
     def close(self):
         if self._h:
