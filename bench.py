@@ -273,10 +273,18 @@ def mixed_frame_errors(n, frames=2, warm=2, form="mixed"):
 #   G2P: posx, posy read, four properties read and written (80 B/P); the four
 #     grids read (32 B/N)
 #   particle advect: posx, posy read and written (32 B/P); u, v read (16 B/N)
-def flip_kernel_bytes(n, p_before, p_after):
+#   P2G, the reduced-precision forms' LDS-binned scatter, counted as the
+#     transfer: posx, posy and the properties read once (d with t 32 B/P,
+#     u or v 24 B/P); src (and src t), weight and the cell bytes written
+#     (26 B/N, 17 B/N). The binned records are the kernels' own traffic.
+def flip_kernel_bytes(n, p_before, p_after, binned=False):
     nd, nu, nv = n * n, (n + 1) * n, n * (n + 1)
+    if binned:
+        p2g = (32 * p_before + 26 * nd) + (24 * p_before + 17 * nu) + (24 * p_before + 17 * nv)
+    else:
+        p2g = (36 * p_before + 30 * nd) + (28 * p_before + 21 * nu) + (28 * p_before + 21 * nv)
     return {
-        "p2g": (36 * p_before + 30 * nd) + (28 * p_before + 21 * nu) + (28 * p_before + 21 * nv),
+        "p2g": p2g,
         "g2p": 80 * p_after + 32 * nd,
         "particle_advect": 32 * p_after + 16 * nd,
     }
@@ -314,7 +322,9 @@ def run_flip(args):
     scene = S.config5_scene()
     F.frand_set_state(0xBA5EBA11)
     objs = [F.SolidBox(*b[1:]) for b in scene["bodies"]]
-    solver = F.FluidSolver(8, n, n, scene["params"], objs)
+    form = args.form or "fp64"
+    prec = {"fp64": F.PREC_FP64, "mixed": F.PREC_MIXED, "refine": F.PREC_REFINE}[form]
+    solver = F.FluidSolver(8, n, n, scene["params"], objs, precision=prec)
     for _ in range(args.warmup):
         solver.update(scene["dt"])
     solver.synchronize()
@@ -326,7 +336,7 @@ def run_flip(args):
     for _ in range(args.steps):
         p0 = solver.particle_count()
         st = solver.update(scene["dt"])
-        for k, v in flip_kernel_bytes(n, p0, st.particle_count).items():
+        for k, v in flip_kernel_bytes(n, p0, st.particle_count, binned=form != "fp64").items():
             byt[k] += v
         solves.append([st.heat.iterations, st.pressure.iterations])
         counts.append(st.particle_count)
@@ -353,10 +363,10 @@ def run_flip(args):
     dom = max(("sweep_fwd", "sweep_bwd", "matvec"), key=lambda k: kern.get(k, {"total_ms": -1})["total_ms"])
     k = kern[dom]
     out = {
-        "metric": "solver 8 (FLIP) steps/s at 1024^2, FP64 drop-in",
+        "metric": f"solver 8 (FLIP) steps/s at 1024^2, {'FP64 drop-in' if form == 'fp64' else form + ' form'}",
         "value": args.steps / wall, "unit": "steps/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
         "ms_per_step": 1e3 * wall / args.steps, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-        "dtype": "f64", "data": "synthetic",
+        "dtype": "f64" if form == "fp64" else "f32+f64", "data": "synthetic",
         "config": {"workload": f"solver8-flip-{n}x{n}-reference-scene", "grid": [n, n], "level": 8,
                    "dt": scene["dt"], "particles": counts, "solves": solves},
         "pcg": {"iterations": prof["iterations"], "ms_total": prof["ms"],

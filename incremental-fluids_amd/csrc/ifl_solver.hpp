@@ -111,6 +111,11 @@ struct Solver {
     uint8_t* pflag = nullptr;
     int *cbuf0 = nullptr, *cbuf1 = nullptr, *cbuf2 = nullptr, *cbuf3 = nullptr;
     uint8_t* cflag = nullptr;
+    // reduced-precision forms: the binned particle records and the
+    // single-precision node accumulators of the LDS-binned fromParticles
+    float4* rec32 = nullptr;
+    float* rec32b = nullptr;
+    float* acc32 = nullptr;
     void* flip_sc = nullptr;    // device FlipScalars
     void* flip_host = nullptr;  // pinned mirror
     void* cub_tmp = nullptr;    // hipcub temporary storage

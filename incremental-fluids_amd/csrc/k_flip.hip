@@ -462,6 +462,273 @@ __global__ void p2g_gather_kernel(P2GArgs a) {
     }
 }
 
+// ------------------------------------------------------------ fromParticles, single precision
+// The reduced-precision forms replace the ordered gather by a scatter into
+// LDS. particlesToGrid's three transfers (d with t, u, v) run together:
+// every particle is binned, for each grid, by the 16x16-cell tile of the
+// cell whose corners it feeds (count, scan, scatter of one 16-byte record per
+// grid: the cell in the tile, the two fractions, the property); one
+// workgroup per tile orders its records by cell in LDS and gathers, for
+// each of its 17x17 nodes, the hat weights and weighted values of the
+// particles of the four cells around it in single precision, then flushes
+// them once: nodes inside the tile belong to it alone (plain
+// stores), nodes on its edge are shared with up to three neighbouring tiles
+// (global atomics). The order of the additions is not fixed, so the sums
+// vary in their last bits from run to run; a node's weight is zero exactly
+// when the double weight is (see p2g32_scatter_kernel).
+constexpr int kP2GTile = 16;
+constexpr int kP2GNodes = kP2GTile + 1;
+
+struct P2G32Args {
+    int n;                   // particles
+    int nq;                  // grids transferred (1, or 3 for particlesToGrid)
+    QtyView q[3];
+    const double* prop[3];
+    const double* prop2;     // temperature with density (grid 0), or null
+    uint8_t* cell2;
+    double* src2;
+    int ntx[3];              // tiles per row of each grid
+    int tile_base[4];        // grid g's tiles are [tile_base[g], tile_base[g + 1])
+    const double* posx;
+    const double* posy;
+    int* tile_cnt;           // all tiles
+    int* tile_start;         // all tiles + 1
+    int* tile_fill;          // all tiles
+    float4* rec;             // per record: cell in the tile (bits), fx, fy, property
+    float* rec2;             // temperature of grid 0's records
+    float* acc;              // per grid: weight, value, value 2 (3 x `stride` each)
+    long stride;
+    double* weight;          // the last grid's weights (the reference's _weight after the transfers)
+};
+
+// cnt[key] += 1 for every lane with `on`, one atomic per distinct key in the
+// wave (neighbouring particles mostly share a tile: per-lane atomics on one
+// address serialise); returns the lane's slot, the counter's old value plus
+// its rank among the lanes of its key. Every lane of the wave must call it.
+IFL_DEV int wave_count(int* cnt, int key, bool on) {
+    const int lane = __lane_id();
+    uint64_t pending = __ballot(on);
+    int slot = 0;
+    while (pending) {
+        const int leader = __ffsll((unsigned long long)pending) - 1;
+        const int lk = __shfl(key, leader);
+        const uint64_t same = __ballot(on && key == lk) & pending;
+        int base = 0;
+        if (lane == leader) base = atomicAdd(&cnt[lk], __popcll(same));
+        base = __shfl(base, leader);
+        if ((same >> lane) & 1ull) slot = base + __popcll(same & ((1ull << lane) - 1ull));
+        pending &= ~same;
+    }
+    return slot;
+}
+
+IFL_DEV int p2g32_key(const P2G32Args& a, int g, int ix, int iy) {
+    return a.tile_base[g] + (iy / kP2GTile) * a.ntx[g] + ix / kP2GTile;
+}
+
+__global__ void p2g32_count_kernel(P2G32Args a) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const bool on = i < a.n;
+    const double px = on ? a.posx[i] : 0.0, py = on ? a.posy[i] : 0.0;
+    for (int g = 0; g < a.nq; g++) {
+        double x, y;
+        int ix, iy;
+        p2g_cell(a.q[g], px, py, x, y, ix, iy);
+        (void)wave_count(a.tile_cnt, p2g32_key(a, g, ix, iy), on);
+    }
+}
+
+// The fractions are taken in double (x - ix is exact) and rounded once; a
+// fraction that would round up to 1 is held at the largest float below it,
+// so every hat factor (1 - f or f) is zero exactly when it is in double.
+__global__ void p2g32_scatter_kernel(P2G32Args a) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const bool on = i < a.n;
+    const double px = on ? a.posx[i] : 0.0, py = on ? a.posy[i] : 0.0;
+    const float below1 = 0x1.fffffep-1f;
+    for (int g = 0; g < a.nq; g++) {
+        double x, y;
+        int ix, iy;
+        p2g_cell(a.q[g], px, py, x, y, ix, iy);
+        const int key = p2g32_key(a, g, ix, iy);
+        const int slot = wave_count(a.tile_fill, key, on);
+        if (!on) continue;
+        const int k = a.tile_start[key] + slot;
+        const float fx = fminf((float)(x - ix), below1), fy = fminf((float)(y - iy), below1);
+        const int local = (ix % kP2GTile) | ((iy % kP2GTile) << 8);
+        a.rec[k] = make_float4(__int_as_float(local), fx, fy, (float)a.prop[g][i]);
+        if (g == 0 && a.prop2) a.rec2[k] = (float)a.prop2[i];
+    }
+}
+
+// One workgroup per tile. The tile's records are taken in chunks of
+// kP2GChunk: each thread loads up to kP2GChunk / 256 of them into registers
+// and counts them per cell (integer LDS atomics give each its slot in its
+// cell), and after a scan of the counts writes them into LDS ordered by
+// cell; then each thread gathers, for its nodes, the records of the four
+// cells around the node and adds their hat weights and weighted values in
+// registers (no floating-point atomics). Measured at 1024^2 (the three
+// grids, ~4.1M particles): 138 us with chunks of 1024 records (most tiles
+// take two), 150 us with 512 and 196 us with 2048 (fewer workgroups per CU),
+// against 460 us with every particle's weights added to the nodes by LDS
+// float atomics.
+constexpr int kP2GChunk = 1024;
+
+__global__ __launch_bounds__(256) void p2g32_tile_kernel(P2G32Args a) {
+    constexpr int NN = kP2GNodes, NC = kP2GTile * kP2GTile;
+    constexpr int R = kP2GChunk / 256;
+    __shared__ float4 srec[kP2GChunk];
+    __shared__ float srec2[kP2GChunk];
+    __shared__ int hist[NC + 1], cstart[NC + 1];
+    const int t = blockIdx.x;
+    const int g = t >= a.tile_base[2] && a.nq > 2 ? 2 : t >= a.tile_base[1] && a.nq > 1 ? 1 : 0;
+    const int lt = t - a.tile_base[g];
+    const int x0 = (lt % a.ntx[g]) * kP2GTile, y0 = (lt / a.ntx[g]) * kP2GTile;
+    const bool two = g == 0 && a.prop2 != nullptr;
+    const int b = a.tile_start[t], cnt = a.tile_cnt[t];
+    // this thread's nodes: tid and tid + 256 (289 nodes)
+    float w[2] = {0.0f, 0.0f}, v[2] = {0.0f, 0.0f}, v2[2] = {0.0f, 0.0f};
+    for (int c0 = 0; c0 < cnt; c0 += kP2GChunk) {
+        const int m = min(kP2GChunk, cnt - c0);
+        for (int k = threadIdx.x; k <= NC; k += blockDim.x) hist[k] = 0;
+        __syncthreads();
+        float4 rec[R];
+        float p2[R];
+        int cell[R], slot[R];
+#pragma unroll
+        for (int r = 0; r < R; r++) {
+            const int k = threadIdx.x + r * 256;
+            cell[r] = -1;
+            slot[r] = 0;
+            rec[r] = k < m ? a.rec[b + c0 + k] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+            p2[r] = two && k < m ? a.rec2[b + c0 + k] : 0.0f;
+        }
+#pragma unroll
+        for (int r = 0; r < R; r++) {
+            const int k = threadIdx.x + r * 256;
+            if (k < m) {
+                const int local = __float_as_int(rec[r].x);
+                cell[r] = (local & 0xFF) + (local >> 8) * kP2GTile;
+                slot[r] = atomicAdd(&hist[cell[r]], 1);
+            }
+        }
+        __syncthreads();
+        if (threadIdx.x < 64) {  // exclusive scan of the 256 counts by one wave
+            const int lane = threadIdx.x;
+            int v4[4], sum = 0;
+#pragma unroll
+            for (int i = 0; i < 4; i++) {
+                v4[i] = hist[lane * 4 + i];
+                sum += v4[i];
+            }
+            int incl = sum;
+#pragma unroll
+            for (int off = 1; off < 64; off <<= 1) {
+                const int o = __shfl_up(incl, off);
+                if (lane >= off) incl += o;
+            }
+            int run = incl - sum;
+#pragma unroll
+            for (int i = 0; i < 4; i++) {
+                cstart[lane * 4 + i] = run;
+                run += v4[i];
+            }
+            if (lane == 63) cstart[NC] = run;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int r = 0; r < R; r++)
+            if (cell[r] >= 0) {
+                const int at = cstart[cell[r]] + slot[r];
+                srec[at] = rec[r];
+                if (two) srec2[at] = p2[r];
+            }
+        __syncthreads();
+        // gather: node (lx, ly) takes the cells (lx - 1 .. lx, ly - 1 .. ly)
+#pragma unroll
+        for (int j = 0; j < 2; j++) {
+            const int node = threadIdx.x + j * 256;
+            if (node >= NN * NN) continue;
+            const int lx = node % NN, ly = node / NN;
+#pragma unroll
+            for (int dy = -1; dy <= 0; dy++)
+#pragma unroll
+                for (int dx = -1; dx <= 0; dx++) {
+                    const int cx = lx + dx, cy = ly + dy;
+                    if (cx < 0 || cy < 0 || cx >= kP2GTile || cy >= kP2GTile) continue;
+                    const int c = cx + cy * kP2GTile;
+                    const int q1 = cstart[c + 1];
+#pragma unroll 4
+                    for (int q = cstart[c]; q < q1; q++) {
+                        const float4 r = srec[q];
+                        const float hx = dx == 0 ? 1.0f - r.y : r.y;  // node ix or ix + 1
+                        const float hy = dy == 0 ? 1.0f - r.z : r.z;
+                        const float kk = hx * hy;
+                        w[j] += kk;
+                        v[j] += kk * r.w;
+                        if (two) v2[j] += kk * srec2[q];
+                    }
+                }
+        }
+        __syncthreads();
+    }
+    const QtyView& q = a.q[g];
+    float* aw = a.acc + 3 * g * a.stride;
+    float* a1 = aw + a.stride;
+    float* a2 = aw + 2 * a.stride;
+#pragma unroll
+    for (int j = 0; j < 2; j++) {
+        const int node = threadIdx.x + j * 256;
+        if (node >= NN * NN) continue;
+        const int lx = node % NN, ly = node / NN;
+        const int gx = x0 + lx, gy = y0 + ly;
+        if (gx >= q.w || gy >= q.h) continue;
+        const long n = gx + (long)gy * q.w;
+        if (lx > 0 && ly > 0 && lx < kP2GTile && ly < kP2GTile) {
+            aw[n] = w[j];  // only this tile's particles reach these nodes
+            a1[n] = v[j];
+            if (two) a2[n] = v2[j];
+        } else if (w[j] != 0.0f) {
+            atomicAdd(&aw[n], w[j]);
+            atomicAdd(&a1[n], v[j]);
+            if (two) atomicAdd(&a2[n], v2[j]);
+        }
+    }
+}
+
+// the division or EMPTY marking of 8:682-687, grid blockIdx.y
+__global__ void p2g32_finish_kernel(P2G32Args a) {
+    const int g = blockIdx.y;
+    const QtyView& q = a.q[g];
+    const long nodes = (long)q.w * q.h;
+    long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nodes) return;
+    const float* aw = a.acc + 3 * g * a.stride;
+    const float w = aw[i], v1 = aw[a.stride + i];
+    if (g == a.nq - 1) a.weight[i] = (double)w;
+    const bool two = g == 0 && a.src2 != nullptr;
+    if (w != 0.0f) {
+        q.src[i] = (double)(v1 / w);
+        if (two) a.src2[i] = (double)(aw[2 * a.stride + i] / w);
+    } else {
+        q.src[i] = (double)v1;
+        if (q.cell[i] == IFL_CELL_FLUID) q.cell[i] = IFL_CELL_EMPTY;
+        if (two) {
+            a.src2[i] = (double)aw[2 * a.stride + i];
+            if (a.cell2[i] == IFL_CELL_FLUID) a.cell2[i] = IFL_CELL_EMPTY;
+        }
+    }
+}
+
+__global__ void fill_i32_kernel(int* p, long n) {
+    long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0;
+}
+__global__ void fill_f32_kernel(float* p, long n) {
+    long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0.0f;
+}
+
 // ------------------------------------------------------------ extrapolate 8:478-651
 // fillSolidMask 8:478-513. ready[i] = 1 for the cells the reference pushes
 // first (8:614-620); `hard` is raised when an EMPTY cell has no fluid
@@ -777,6 +1044,11 @@ int flip_alloc(Solver* s) {
     need = std::max(need, b);
     s->cub_bytes = need;
     IFL_HIP(hipMalloc(&s->cub_tmp, need));
+    if (s->cfg.precision != IFL_PREC_FP64) {
+        IFL_HIP(hipMalloc((void**)&s->rec32, 3 * mp * sizeof(float4)));
+        IFL_HIP(hipMalloc((void**)&s->rec32b, 3 * mp * sizeof(float)));
+        IFL_HIP(hipMalloc((void**)&s->acc32, 9 * cells * sizeof(float)));
+    }
     return IFL_OK;
 }
 
@@ -784,7 +1056,7 @@ void flip_free(Solver* s) {
     auto F = [](void* p) { if (p) (void)hipFree(p); };
     F(s->pkey); F(s->pkey2); F(s->pidx); F(s->pidx2); F(s->pflag);
     F(s->cbuf0); F(s->cbuf1); F(s->cbuf2); F(s->cbuf3); F(s->cflag);
-    F(s->flip_sc); F(s->cub_tmp);
+    F(s->flip_sc); F(s->cub_tmp); F(s->rec32); F(s->rec32b); F(s->acc32);
     if (s->flip_host) (void)hipHostFree(s->flip_host);
 }
 
@@ -919,7 +1191,61 @@ int k_seed(Solver* s) {
 // fromParticles for quantity qi; with qi == IFL_Q_D the temperature, which
 // lives on the same grid, is transferred in the same pass (the weights of a
 // node depend only on the positions).
+// fromParticles of the grids qis[0..nq) (with t on d's grid when with_t)
+static int k_from_particles32(Solver* s, const int* qis, int nq, bool with_t) {
+    P2G32Args a;
+    a.n = s->particle_count;
+    a.nq = nq;
+    a.tile_base[0] = 0;
+    long most = 0;
+    for (int g = 0; g < 3; g++) {
+        const int qi = g < nq ? qis[g] : qis[0];
+        a.q[g] = s->q[qi];
+        a.prop[g] = s->prop[qi];
+        a.ntx[g] = (a.q[g].w + kP2GTile - 1) / kP2GTile;
+        const int nty = (a.q[g].h + kP2GTile - 1) / kP2GTile;
+        a.tile_base[g + 1] = a.tile_base[g] + (g < nq ? a.ntx[g] * nty : 0);
+        most = std::max(most, (long)a.q[g].w * a.q[g].h);
+    }
+    const int ntiles = a.tile_base[3];
+    const bool t0 = with_t && qis[0] == IFL_Q_D;
+    a.prop2 = t0 ? s->prop[IFL_Q_T] : nullptr;
+    a.cell2 = t0 ? s->q[IFL_Q_T].cell : nullptr;
+    a.src2 = t0 ? s->q[IFL_Q_T].src : nullptr;
+    a.posx = s->posx;
+    a.posy = s->posy;
+    a.tile_cnt = s->cbuf0;
+    a.tile_start = s->cbuf1;
+    a.tile_fill = s->cbuf2;
+    a.rec = s->rec32;
+    a.rec2 = s->rec32b;
+    a.acc = s->acc32;
+    a.stride = (long)(s->W + 1) * (s->H + 1) + 1;
+    a.weight = s->weight;
+    IFL_TRY(prof_begin(s, IFL_KERNEL_P2G, -1));
+    fill_i32_kernel<<<grid_blocks(ntiles + 1, 256), 256, 0, s->stream>>>(s->cbuf0, ntiles + 1);
+    IFL_LAUNCH_CHECK();
+    fill_i32_kernel<<<grid_blocks(ntiles, 256), 256, 0, s->stream>>>(s->cbuf2, ntiles);
+    IFL_LAUNCH_CHECK();
+    fill_f32_kernel<<<grid_blocks(3L * nq * a.stride, 256), 256, 0, s->stream>>>(s->acc32, 3L * nq * a.stride);
+    IFL_LAUNCH_CHECK();
+    if (a.n > 0) {
+        p2g32_count_kernel<<<grid_blocks(a.n, 256), 256, 0, s->stream>>>(a);
+        IFL_LAUNCH_CHECK();
+        IFL_TRY(cub_exclusive_sum(s, s->cbuf0, s->cbuf1, ntiles + 1));
+        p2g32_scatter_kernel<<<grid_blocks(a.n, 256), 256, 0, s->stream>>>(a);
+        IFL_LAUNCH_CHECK();
+        p2g32_tile_kernel<<<ntiles, 256, 0, s->stream>>>(a);
+        IFL_LAUNCH_CHECK();
+    }
+    p2g32_finish_kernel<<<dim3(grid_blocks(most, 256), nq), 256, 0, s->stream>>>(a);
+    IFL_LAUNCH_CHECK();
+    IFL_TRY(prof_end(s));
+    return IFL_OK;
+}
+
 int k_from_particles(Solver* s, int qi, bool with_t) {
+    if (s->cfg.precision != IFL_PREC_FP64) return k_from_particles32(s, &qi, 1, with_t);
     const QtyView& q = s->q[qi];
     long cells = (long)q.w * q.h;
     int n = s->particle_count;
@@ -974,6 +1300,21 @@ int k_extrapolate_flip(Solver* s, int qi) {
 
 // particlesToGrid 8:916-927
 int k_particles_to_grid(Solver* s) {
+    if (s->cfg.precision != IFL_PREC_FP64) {
+        // the three transfers together (each extrapolation reads and writes
+        // only its own grid, and no transfer reads a grid), then the
+        // extrapolations in the reference's order
+        const int qis[3] = {IFL_Q_D, IFL_Q_U, IFL_Q_V};
+        IFL_TRY(k_from_particles32(s, qis, 3, true));
+        IFL_TRY(k_extrapolate_flip(s, IFL_Q_D));
+        IFL_TRY(k_extrapolate_flip(s, IFL_Q_T));
+        IFL_TRY(k_extrapolate_flip(s, IFL_Q_U));
+        IFL_TRY(k_extrapolate_flip(s, IFL_Q_V));
+        IFL_TRY(k_count_particles(s));
+        IFL_TRY(k_prune(s));
+        IFL_TRY(k_seed(s));
+        return IFL_OK;
+    }
     IFL_TRY(k_from_particles(s, IFL_Q_D, true));
     IFL_TRY(k_extrapolate_flip(s, IFL_Q_D));
     IFL_TRY(k_extrapolate_flip(s, IFL_Q_T));
