@@ -50,6 +50,8 @@ def parse():
                          "batch: 256 independent solver-4 instances of mixed sizes")
     ap.add_argument("--threads", type=int, default=16, help="batch: host threads stepping the instances")
     ap.add_argument("--instances", type=int, default=256, help="batch: number of instances")
+    ap.add_argument("--batch-level", type=int, choices=[4, 8], default=4,
+                    help="batch: solver 4 with the drawn sizes (config 4), or solver 8 (FLIP) at 256^2 (config 5)")
     ap.add_argument("--batch-mode", choices=["fused", "streams"], default="fused",
                     help="batch: one ifl_batch (PCG iterations fused over the instances) or one stream per instance")
     ap.add_argument("--form", choices=["fp64", "mixed", "refine"], default=None,
@@ -413,7 +415,16 @@ def _cpu_baseline_batch(insts, sample):
                 sample=f"first update() of the first {sample} instances ({cells} cells) on one host core: {secs:.1f} s")
 
 
-def measure_batch(n_inst, steps, warmup, mode="fused", threads=16):
+def flip_instances(n, size=256):
+    """Config 5's batched part: n instances of the reference's FLIP scene
+    (8:1469-1513) at size x size."""
+    import ifl_scenes as S
+    sc = S.config5_scene()
+    return [dict(w=size, h=size, params=sc["params"], bodies=sc["bodies"], inflows=[], dt=sc["dt"])
+            for _ in range(n)]
+
+
+def measure_batch(n_inst, steps, warmup, mode="fused", threads=16, level=4):
     """Config 4: independent solver-4 instances (sides drawn from 64..512,
     one static box and one inflow each, ifl_scenes.config4_instances),
     stepped as one ifl_batch (mode "fused") or each on its own stream from
@@ -424,8 +435,10 @@ def measure_batch(n_inst, steps, warmup, mode="fused", threads=16):
     import threading
     import ifl_fluids as F
     import ifl_scenes as S
-    insts = S.config4_instances(n_inst)
-    solvers = [F.FluidSolver(4, i["w"], i["h"], i["params"], [F.SolidBox(*b[1:]) for b in i["bodies"]])
+    insts = S.config4_instances(n_inst) if level == 4 else flip_instances(n_inst)
+    if level == 8:
+        F.frand_set_state(0xBA5EBA11)
+    solvers = [F.FluidSolver(level, i["w"], i["h"], i["params"], [F.SolidBox(*b[1:]) for b in i["bodies"]])
                for i in insts]
     cells = sum(i["w"] * i["h"] for i in insts)
     nt = max(1, threads)
@@ -482,7 +495,7 @@ def measure_batch(n_inst, steps, warmup, mode="fused", threads=16):
             p = batch.profile_read(kid)
             if p["launches"] == 0 or p["ms"] <= 0:
                 continue
-            byts = BATCH_BYTES_PER_CELL[name] * p["cell_iterations"]
+            byts = (BATCH_BYTES_PER_CELL if level == 4 else KERNEL_BYTES_PER_CELL)[name] * p["cell_iterations"]
             kern[name] = {"launches": p["launches"], "total_ms": p["ms"], "avg_us": 1e3 * p["ms"] / p["launches"],
                           "cell_iterations": p["cell_iterations"], "bytes": byts,
                           "achieved_GBps": byts / (p["ms"] * 1e-3) / 1e9}
@@ -502,7 +515,8 @@ def measure_batch(n_inst, steps, warmup, mode="fused", threads=16):
         k = kern[dom]
         out["roofline"] = {"bound": "hbm", "achieved": k["achieved_GBps"], "peak": PEAK_HBM_GBS, "unit": "GB/s",
                            "frac": k["achieved_GBps"] / PEAK_HBM_GBS, "traffic": None,
-                           "kernel": f"fused {dom} over all instances: {BATCH_BYTES_PER_CELL[dom]:.0f} B/cell x the "
+                           "kernel": f"fused {dom} over all instances: "
+                                     f"{(BATCH_BYTES_PER_CELL if level == 4 else KERNEL_BYTES_PER_CELL)[dom]:.0f} B/cell x the "
                                      f"cells of the instances iterating in each launch, over the launches' summed "
                                      f"time (HIP events on the batch's stream, one profiled step)"}
     return out
@@ -511,24 +525,28 @@ def measure_batch(n_inst, steps, warmup, mode="fused", threads=16):
 def run_batch(args):
     """`--workload batch`: config 4 as the headline (cell-updates/s)."""
     try:
-        m = measure_batch(args.instances, args.steps, args.warmup, args.batch_mode, args.threads)
+        m = measure_batch(args.instances, args.steps, args.warmup, args.batch_mode, args.threads, args.batch_level)
     except RuntimeError as e:
         print(json.dumps({"error": str(e)}))
         sys.exit(1)
     insts = m.pop("insts")
     out = {
-        "metric": "cell-updates/s, 256 independent solver-4 instances of mixed sizes, FP64 drop-in",
+        "metric": (f"cell-updates/s, {m['instances']} independent solver-4 instances of mixed sizes, FP64 drop-in"
+                   if args.batch_level == 4 else
+                   f"cell-updates/s, {m['instances']} independent solver-8 (FLIP) instances at 256^2, FP64 drop-in"),
         "value": m["value"], "unit": "cell-updates/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": m["ms_per_step"], "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "f64", "data": "synthetic",
-        "config": {"workload": f"solver4-batch-{m['instances']}-instances-64to512", "instances": m["instances"],
+        "config": {"workload": (f"solver4-batch-{m['instances']}-instances-64to512" if args.batch_level == 4
+                                else f"solver8-flip-batch-{m['instances']}-instances-256x256"),
+                   "instances": m["instances"],
                    "cells": m["cells"], "mode": m["mode"], "threads": m["threads"], "streams": m["streams"],
                    "pressure_iterations_per_step": m["pressure_iterations_per_step"]},
     }
     if "kernels" in m:
         out["kernels"] = m["kernels"]
         out["roofline"] = m["roofline"]
-    if not args.no_cpu_baseline:
+    if not args.no_cpu_baseline and args.batch_level == 4:
         try:
             out["cpu_baseline"] = cpu_baseline_batch(insts, 16)
         except Exception as e:

@@ -204,12 +204,14 @@ int ifl_set_particle_count(ifl_solver* s, int count);
 int ifl_init_particles(ifl_solver* s);
 
 /* ---- batches: independent solvers stepped together ----
- * The solvers (one level, 3..7, one device, FP64 form, tree-ordered dots)
+ * The solvers (one level, 3..8, one device, FP64 form, tree-ordered dots)
  * keep their own state; ifl_batch_update runs FluidSolver::update(dt) on all
  * of them (the reference steps each in turn, 7:1110-1126), with each PCG
  * iteration of their project() calls one launch per phase for the whole
- * batch. stats receives n entries. No counterpart in the reference: a
- * caller with many small simulations would otherwise step them one by one. */
+ * batch. Level 8 solvers draw frand() in batch order within each step, as
+ * the same solvers stepped one after the other would. stats receives n
+ * entries. No counterpart in the reference: a caller with many small
+ * simulations would otherwise step them one by one. */
 typedef struct ifl_batch ifl_batch;
 int ifl_batch_create(ifl_solver* const* solvers, int n, ifl_batch** out);
 int ifl_batch_update(ifl_batch* batch, double dt, ifl_step_stats* stats);

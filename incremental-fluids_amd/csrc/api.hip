@@ -898,9 +898,9 @@ int ifl_batch_create(ifl_solver* const* handles, int n, ifl_batch** out) {
     for (int i = 0; i < n; i++) {
         const Solver* s = handles[i] ? &handles[i]->s : nullptr;
         if (!s) return IFL_ERR_ARG;
-        if (s->level != s0->level || s->level == 8 || s->device != s0->device ||
+        if (s->level != s0->level || s->device != s0->device ||
             s->cfg.precision != IFL_PREC_FP64 || s->cfg.dot_order != IFL_DOT_TREE) {
-            g_err = "ifl_batch_create: solvers of one level (3-7) on one device, FP64 form, tree-ordered dots";
+            g_err = "ifl_batch_create: solvers of one level on one device, FP64 form, tree-ordered dots";
             return IFL_ERR_UNSUPPORTED;
         }
         for (int k = 0; k < i; k++)

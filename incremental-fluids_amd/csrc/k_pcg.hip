@@ -2154,8 +2154,8 @@ int k_project_batch(Solver* const* ss, int n, hipStream_t stream, BatchDev** dev
                     BatchProfile* prof) {
     Solver* s0 = ss[0];
     const int lv = s0->level;
-    if (lv == 8 || s0->cfg.precision != IFL_PREC_FP64 || s0->cfg.dot_order != IFL_DOT_TREE) {
-        s0->err = "batched solves take levels 3-7, the FP64 form and tree-ordered dots";
+    if (s0->cfg.precision != IFL_PREC_FP64 || s0->cfg.dot_order != IFL_DOT_TREE) {
+        s0->err = "batched solves take the FP64 form and tree-ordered dots";
         return IFL_ERR_UNSUPPORTED;
     }
     BatchDev* d = *dev;

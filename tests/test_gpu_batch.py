@@ -98,3 +98,48 @@ def test_fused_batch_equals_single_solvers(level):
     print(f"level {level}: (heat, pressure) iterations per step and instance {iters_b}; statuses {stat_b}",
           flush=True)
     assert len(counts) > 1  # the instances really converge at different iterations
+
+
+def test_fused_batch_flip_16x256():
+    """Solver 8 in the batch: 16 instances of the reference's FLIP scene at
+    256x256 (about 4.2M particles in all) stepped as one batch, against the
+    same 16 solvers stepped one after the other from the same frand() state
+    (the batch draws in instance order within each step): identical
+    statuses, iteration counts, particle counts, grid fields and particles."""
+    import ifl_fluids as F
+    import ifl_scenes as S
+    sc = S.config5_scene()
+    n, size, steps = 16, 256, 2
+
+    def make():
+        return F.FluidSolver(8, size, size, sc["params"], [F.SolidBox(*b[1:]) for b in sc["bodies"]])
+    seed = 0xBA5EBA11
+    F.frand_set_state(seed)
+    batched = [make() for _ in range(n)]
+    batch = F.Batch(batched)
+    rows_b = []
+    for _ in range(steps):
+        st = batch.update(sc["dt"])
+        rows_b.append([(x.heat.iterations, x.pressure.iterations, x.heat.status, x.pressure.status,
+                        x.particle_count) for x in st])
+    batch.close()
+    state_b = F.frand_state()
+    F.frand_set_state(seed)
+    singles = [make() for _ in range(n)]
+    rows_s = []
+    for _ in range(steps):
+        row = []
+        for s in singles:
+            x = s.update(sc["dt"])
+            row.append((x.heat.iterations, x.pressure.iterations, x.heat.status, x.pressure.status,
+                        x.particle_count))
+        rows_s.append(row)
+    assert rows_s == rows_b
+    assert F.frand_state() == state_b
+    fields = ol.state_fields(8)
+    for k in range(n):
+        bad = compare_fields(singles[k], batched[k], fields, where=f"instance {k}")
+        assert not bad, "\n".join(bad[:6])
+        singles[k].close()
+        batched[k].close()
+    print(f"16 x 256^2 FLIP batch: (heat, pressure, statuses, particles) per step {rows_b[-1][:3]} ...", flush=True)
