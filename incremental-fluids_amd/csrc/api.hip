@@ -1,6 +1,7 @@
 // C ABI of libifl.so (declared in include/ifl.h): solver lifetime, state
 // access, single stages and FluidSolver::update() for solvers 3..8.
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -38,6 +39,20 @@ static __device__ __align__(256) uint8_t scratch[kScratchBytes];
 constexpr int kMaxDevices = 64;
 static std::mutex g_scratch_mu;
 static void* g_scratch_dev[kMaxDevices];
+
+// Calls stepping solvers on each device right now (ifl_update, ifl_stage,
+// ifl_batch_update, from any thread): a sweep launched while another call
+// is under way deals its strips upstream first (strip_of_block).
+static std::atomic<int> g_busy[kMaxDevices];
+struct BusyGuard {
+    std::atomic<int>* c;
+    explicit BusyGuard(int device) : c(device >= 0 && device < kMaxDevices ? &g_busy[device] : nullptr) {
+        if (c) c->fetch_add(1, std::memory_order_relaxed);
+    }
+    ~BusyGuard() {
+        if (c) c->fetch_sub(1, std::memory_order_relaxed);
+    }
+};
 
 // With `device` current. Idempotent; the address is kept per device.
 static int scratch_init(Solver* s) {
@@ -261,6 +276,7 @@ int init_solver(Solver* s) {
     int cus = 0;
     IFL_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, s->device));
     s->num_cus = cus > 0 ? cus : 1;
+    s->busy = s->device >= 0 && s->device < kMaxDevices ? &g_busy[s->device] : nullptr;
     int W = s->W, H = s->H, lv = s->level;
     // FluidSolver ctor 7:953-958: d, t at cell centres, u, v on faces
     const int qw[4] = {W, W, W + 1, W};
@@ -742,6 +758,7 @@ int ifl_update(ifl_solver* h, double dt, ifl_step_stats* stats) {
     if (!h) return IFL_ERR_ARG;
     Solver* s = &h->s;
     IFL_HIP(hipSetDevice(s->device));
+    BusyGuard busy(s->device);
     return update_grid(s, dt, stats);
 }
 
@@ -927,6 +944,7 @@ int ifl_batch_update(ifl_batch* b, double dt, ifl_step_stats* stats) {
     if (!b) return IFL_ERR_ARG;
     const int n = (int)b->solvers.size();
     if (hipSetDevice(b->device) != hipSuccess) return IFL_ERR_HIP;
+    BusyGuard busy(b->device);
     // The per-solver stages run on each solver's own stream, so that the
     // small kernels of different solvers overlap; the batched iterations run
     // on the batch's stream, joined to the solver streams by events
@@ -999,6 +1017,7 @@ int ifl_stage(ifl_solver* h, int stage, int iarg, double darg, ifl_solve_stats* 
     Solver* s = &h->s;
     int lv = s->level;
     IFL_HIP(hipSetDevice(s->device));
+    BusyGuard busy(s->device);
     bool qok = iarg >= 0 && iarg < 4 && !(iarg == IFL_Q_T && lv < 6);
     int rc = IFL_OK;
     switch (stage) {

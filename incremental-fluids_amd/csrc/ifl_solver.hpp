@@ -6,6 +6,7 @@
 
 #include <cstring>
 #include <string>
+#include <atomic>
 #include <vector>
 
 #include "ifl_common.hpp"
@@ -142,6 +143,7 @@ struct Solver {
     bool border_pending = false;  // step_flags[1] holds this step's border flag
     size_t lds_bytes = 65536;  // dynamic LDS available to one workgroup
     int num_cus = 1;           // compute units: the strips of one sweep launch that can be resident at once
+    std::atomic<int>* busy = nullptr;  // calls stepping solvers on this device right now
 };
 
 // error helpers

@@ -43,16 +43,25 @@ IFL_DEV void st_relaxed(double* p, uint64_t v) {
 IFL_DEV double as_double(uint64_t v) { return __longlong_as_double((long long)v); }
 IFL_DEV uint64_t as_bits(double v) { return (uint64_t)__double_as_longlong(v); }
 
-// Strips are assigned so that consecutive strips share an XCD (workgroups
-// are dealt round-robin over the 8 XCDs): the edge hand-off then stays in
-// one L2. `n` strips from `first` are launched together.
-IFL_DEV int strip_of_block(int b, int first, int n) {
+// Two ways to deal a launch's strips to its workgroups (blocks are dealt
+// round robin over the 8 XCDs and dispatched in block order):
+//  order 0: consecutive strips share an XCD (strip (b mod 8) * per + b / 8),
+//           the edge hand-offs staying in one L2. A strip's upstream
+//           neighbour may then have a higher block id, so every strip of the
+//           launch must be resident at once;
+//  order 1: upstream first (block b is strip b of a forward sweep, strip
+//           n - 1 - b of a backward one), so the lowest unfinished strip can
+//           always run and no strip needs the others resident.
+// Order 1 is used when the strips outnumber the CUs or when another solver
+// is stepping on the device at the same time; alone, order 0 measured
+// faster at 2048^2 (forward / backward 384 / 356 us, against 388 / 400 us).
+IFL_DEV int strip_of_block(int b, int n, bool reverse, int order) {
+    if (order) return b < n ? (reverse ? n - 1 - b : b) : -1;
     int per = (n + 7) >> 3;
     int s = (b & 7) * per + (b >> 3);
-    return ((b >> 3) < per && s < n) ? first + s : -1;
+    return ((b >> 3) < per && s < n) ? s : -1;
 }
-static int sweep_blocks(int n) { return 8 * ((n + 7) / 8); }
-
+static int sweep_blocks(int n, int order) { return order ? n : 8 * ((n + 7) / 8); }
 
 // ---------------------------------------------------------------- streamed sweeps
 // One strip runs as a workgroup of six waves. Waves w and w + 4 of a
@@ -119,6 +128,7 @@ struct StreamArgs {
     int* timeout;
     double tau, sigma;
     int strip0, nlaunch;    // the strips of this launch
+    int order;              // how strips are dealt to blocks (strip_of_block)
     int nb;                 // ring depth in blocks
     int slot_bytes;         // bytes of one ring slot
     long long* trace;       // diagnostics: kTraceStamps stamps per strip and block, or null
@@ -760,7 +770,7 @@ IFL_DEV void sweep_body(const StreamArgs& a, int j) {
 
 template <int KIND, int MODE, int LV, int NG, typename T, bool TR>
 __global__ __launch_bounds__(64 * kWaves) void stream_sweep_kernel(StreamArgs a) {
-    const int j = strip_of_block(blockIdx.x, a.strip0, a.nlaunch);
+    const int j = strip_of_block(blockIdx.x, a.nlaunch, KIND == SW_BWD, a.order);
     if (j < 0) return;
     sweep_body<KIND, MODE, LV, NG, T, TR>(a, j);
 }
@@ -1128,16 +1138,11 @@ static int launch_stream_k(Solver* s, StreamArgs a) {
                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)shmem));
         attr_set.store(true, std::memory_order_release);
     }
-    // A strip waits for its upstream neighbour, so every strip of a launch
-    // must be resident at once (one workgroup per CU): taller grids run as
-    // several launches in sweep order, the edges crossing through memory.
-    const int ns = s->sk.nstrips, chunk = std::max(8, s->num_cus / 8 * 8);
-    for (int done = 0; done < ns; done += chunk) {
-        a.nlaunch = std::min(chunk, ns - done);
-        a.strip0 = KIND == SW_BWD ? ns - done - a.nlaunch : done;
-        stream_sweep_kernel<KIND, MODE, LV, NG, T, TR><<<sweep_blocks(a.nlaunch), 64 * kWaves, shmem, s->stream>>>(a);
-        IFL_LAUNCH_CHECK();
-    }
+    a.nlaunch = s->sk.nstrips;
+    a.strip0 = 0;
+    a.order = (a.nlaunch > s->num_cus || (s->busy && s->busy->load(std::memory_order_relaxed) > 1)) ? 1 : 0;
+    stream_sweep_kernel<KIND, MODE, LV, NG, T, TR><<<sweep_blocks(a.nlaunch, a.order), 64 * kWaves, shmem, s->stream>>>(a);
+    IFL_LAUNCH_CHECK();
     return IFL_OK;
 }
 

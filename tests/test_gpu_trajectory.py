@@ -29,6 +29,9 @@ CASES = [
     (6, 72, 40, dict(bodies=MULTI), 8),
     (7, 64, 64, None, 8),
     (7, 72, 40, dict(bodies=MULTI), 8),
+    # 257 strips, more than the GPU's CUs: one launch per sweep with the
+    # strips dealt upstream first
+    (3, 32, 16448, None, 2),
 ]
 
 
