@@ -414,7 +414,7 @@ int update_part(Solver* s, int after, double dt, int* next) {
         if (lv == 3) {
             IFL_TRY(k_build_rhs(s));
             IFL_TRY(k_build_pressure(s, dt));
-            IFL_TRY(k_build_precon(s));
+            if (!s->batched) IFL_TRY(k_build_precon(s));
             *next = SOLVE_PRESSURE;
             return IFL_OK;
         }
@@ -437,7 +437,7 @@ int update_part(Solver* s, int after, double dt, int* next) {
         if (lv >= 6) {
             IFL_TRY(k_copy_t_to_r(s));
             IFL_TRY(k_build_heat(s, dt));
-            IFL_TRY(k_build_precon(s));
+            if (!s->batched) IFL_TRY(k_build_precon(s));
             *next = SOLVE_HEAT;
             return IFL_OK;
         }
@@ -452,7 +452,7 @@ int update_part(Solver* s, int after, double dt, int* next) {
         IFL_TRY(k_build_rhs(s));
         if (lv >= 7) IFL_TRY(k_compute_densities(s));
         IFL_TRY(k_build_pressure(s, dt));
-        IFL_TRY(k_build_precon(s));
+        if (!s->batched) IFL_TRY(k_build_precon(s));
         *next = SOLVE_PRESSURE;
         return IFL_OK;
     }
@@ -952,6 +952,12 @@ int ifl_batch_update(ifl_batch* b, double dt, ifl_step_stats* stats) {
     const ifl_solve_stats none = {IFL_SOLVE_NOT_RUN, 0, 0.0};
     std::vector<ifl_solve_stats> heat(n, none), press(n, none);
     int rc = IFL_OK, after = SOLVE_NONE;
+    // buildPreconditioner runs batched inside k_project_batch
+    struct Batched {
+        std::vector<Solver*>& v;
+        explicit Batched(std::vector<Solver*>& x) : v(x) { for (Solver* s : v) s->batched = true; }
+        ~Batched() { for (Solver* s : v) s->batched = false; }
+    } batched(b->solvers);
     while (rc == IFL_OK) {
         int next = SOLVE_NONE;
         for (int i = 0; i < n && rc == IFL_OK; i++) {

@@ -144,6 +144,7 @@ struct Solver {
     size_t lds_bytes = 65536;  // dynamic LDS available to one workgroup
     int num_cus = 1;           // compute units: the strips of one sweep launch that can be resident at once
     std::atomic<int>* busy = nullptr;  // calls stepping solvers on this device right now
+    bool batched = false;      // stepped by ifl_batch_update: the batch builds the MIC(0) factors
 };
 
 // error helpers

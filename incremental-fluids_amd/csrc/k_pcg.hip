@@ -1211,7 +1211,7 @@ static int launch_bwd(Solver* s, long long* trace = nullptr) {
 
 static long skew_n(Solver* s) { return (long)s->sk.nstrips * s->sk.strip_sz; }
 
-static int launch_precon(Solver* s, long long* trace) {
+static StreamArgs precon_args(Solver* s) {
     StreamArgs a = stream_args(s, SW_PRECON);
     a.in[0] = s->adiag; a.in[1] = s->aplusx; a.in[2] = s->aplusy;
     a.out[0] = s->precon;
@@ -1222,6 +1222,11 @@ static int launch_precon(Solver* s, long long* trace) {
         a.out[4] = s->cx32;
         a.out[5] = s->cy32;
     }
+    return a;
+}
+
+static int launch_precon(Solver* s, long long* trace) {
+    StreamArgs a = precon_args(s);
     a.trace = trace;
     return launch_stream<SW_PRECON, 0, 3, 3, double>(s, a);
 }
@@ -1891,10 +1896,13 @@ struct BatchInst {
 
 struct BatchDev {
     int n = 0, nvblk = 0, nsblk = 0;
-    size_t shmem_f = 0, shmem_b = 0;
+    size_t shmem_f = 0, shmem_b = 0, shmem_p = 0, shmem_f0 = 0, shmem_b0 = 0;
     BatchInst* inst = nullptr;    // n
-    StreamArgs* fwd = nullptr;    // n
+    StreamArgs* fwd = nullptr;    // n: the sweeps inside the PCG loop
     StreamArgs* bwd = nullptr;    // n
+    StreamArgs* pre = nullptr;    // n: buildPreconditioner
+    StreamArgs* fwd0 = nullptr;   // n: applyPreconditioner at the entry of project()
+    StreamArgs* bwd0 = nullptr;   // n
     int2* vmap = nullptr;         // nvblk
     int2* fmap = nullptr;         // nsblk
     int2* bmap = nullptr;         // nsblk
@@ -1918,7 +1926,7 @@ struct BatchDev {
 void batch_dev_free(BatchDev* d) {
     if (!d) return;
     auto F = [](void* p) { if (p) (void)hipFree(p); };
-    F(d->inst); F(d->fwd); F(d->bwd); F(d->vmap); F(d->fmap); F(d->bmap); F(d->states); F(d->st_ptrs);
+    F(d->inst); F(d->fwd); F(d->bwd); F(d->pre); F(d->fwd0); F(d->bwd0); F(d->vmap); F(d->fmap); F(d->bmap); F(d->states); F(d->st_ptrs);
     if (d->states_host) (void)hipHostFree(d->states_host);
     for (hipEvent_t e : d->ready) (void)hipEventDestroy(e);
     for (hipEvent_t e : d->ev_pool) (void)hipEventDestroy(e);
@@ -1956,11 +1964,44 @@ __global__ __launch_bounds__(256) void batch_fin_sigma_kernel(const BatchInst* t
     fin_sigma_body(b.v, b.nstrips, 0, 0, b.tol, b.limit);
 }
 
-template <int KIND, int LV, int NG>
+template <int KIND, int MODE, int LV, int NG>
 __global__ __launch_bounds__(64 * kWaves) void batch_sweep_kernel(const StreamArgs* t, const int2* map) {
     const int2 m = map[blockIdx.x];
     const StreamArgs a = t[m.x];
-    sweep_body<KIND, 1, LV, NG, double, false>(a, m.y);
+    sweep_body<KIND, MODE, LV, NG, double, false>(a, m.y);
+}
+
+// the entry of project() (7:854-864) for every instance: the state reset,
+// p = 0, s = z after the entry sweeps, then sigma and the early-exit test
+__global__ void batch_reset_kernel(const BatchInst* t, int n) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    PcgState* st = t[i].v.st;
+    st->zmean = 0.0;
+    st->rmean = 0.0;
+    st->sigma = st->alpha = st->beta = 0.0;
+    st->max_err = 0.0;
+    st->dot_tmp = 0.0;
+    st->iter = 0;
+    st->done = 0;
+    st->status = IFL_SOLVE_NOT_RUN;
+    st->printed_iter = 0;
+}
+__global__ __launch_bounds__(256) void batch_zero_p_kernel(const BatchInst* t, const int2* map) {
+    const int2 m = map[blockIdx.x];
+    const VecArgs& a = t[m.x].v;
+    const long i = (long)m.y * blockDim.x + threadIdx.x;
+    if (i < (long)a.sk.nstrips * a.sk.strip_sz) a.p[i] = 0.0;
+}
+__global__ __launch_bounds__(256) void batch_copy_s_kernel(const BatchInst* t, const int2* map) {
+    const int2 m = map[blockIdx.x];
+    const VecArgs& a = t[m.x].v;
+    const long i = (long)m.y * blockDim.x + threadIdx.x;
+    if (i < (long)a.sk.nstrips * a.sk.strip_sz) a.s[i] = a.z[i];
+}
+__global__ __launch_bounds__(256) void batch_fin_sigma_init_kernel(const BatchInst* t) {
+    const BatchInst& b = t[blockIdx.x];
+    fin_sigma_body(b.v, b.nstrips, 0, 1, b.tol, b.limit);
 }
 
 __global__ void batch_gather_kernel(PcgState* const* st, int n, PcgState* out) {
@@ -1979,10 +2020,16 @@ template <int LV, int NGF, int NGB>
 int batch_attrs(Solver* s, BatchDev* d) {
     static std::atomic<bool> set{false};
     if (!set.load(std::memory_order_acquire)) {
-        IFL_HIP(hipFuncSetAttribute((const void*)batch_sweep_kernel<SW_FWD, LV, NGF>,
+        IFL_HIP(hipFuncSetAttribute((const void*)batch_sweep_kernel<SW_FWD, 1, LV, NGF>,
                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)d->shmem_f));
-        IFL_HIP(hipFuncSetAttribute((const void*)batch_sweep_kernel<SW_BWD, LV, NGB>,
+        IFL_HIP(hipFuncSetAttribute((const void*)batch_sweep_kernel<SW_BWD, 1, LV, NGB>,
                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)d->shmem_b));
+        IFL_HIP(hipFuncSetAttribute((const void*)batch_sweep_kernel<SW_PRECON, 0, 3, 3>,
+                                    hipFuncAttributeMaxDynamicSharedMemorySize, (int)d->shmem_p));
+        IFL_HIP(hipFuncSetAttribute((const void*)batch_sweep_kernel<SW_FWD, 0, LV, NGF - 1>,
+                                    hipFuncAttributeMaxDynamicSharedMemorySize, (int)d->shmem_f0));
+        IFL_HIP(hipFuncSetAttribute((const void*)batch_sweep_kernel<SW_BWD, 0, LV, NGB>,
+                                    hipFuncAttributeMaxDynamicSharedMemorySize, (int)d->shmem_b0));
         set.store(true, std::memory_order_release);
     }
     return IFL_OK;
@@ -1992,7 +2039,7 @@ template <int LV, int NGF, int NGB>
 int batch_build(Solver* const* ss, int n, BatchDev* d) {
     Solver* s0 = ss[0];
     std::vector<BatchInst> inst(n);
-    std::vector<StreamArgs> fwd(n), bwd(n);
+    std::vector<StreamArgs> fwd(n), bwd(n), pre(n), fwd0(n), bwd0(n);
     std::vector<int2> vmap, fmap, bmap;
     std::vector<PcgState*> sts(n);
     for (int i = 0; i < n; i++) {
@@ -2011,6 +2058,15 @@ int batch_build(Solver* const* ss, int n, BatchDev* d) {
         IFL_TRY((ring_setup<SW_BWD, 1, LV, NGB, double>(s, bwd[i], &shb)));
         d->shmem_f = shf;
         d->shmem_b = shb;
+        pre[i] = precon_args(s);
+        fwd0[i] = fwd_args<0, double>(s);
+        bwd0[i] = bwd_args<0, double>(s);
+        IFL_TRY((ring_setup<SW_PRECON, 0, 3, 3, double>(s, pre[i], &shf)));
+        d->shmem_p = shf;
+        IFL_TRY((ring_setup<SW_FWD, 0, LV, NGF - 1, double>(s, fwd0[i], &shf)));
+        d->shmem_f0 = shf;
+        IFL_TRY((ring_setup<SW_BWD, 0, LV, NGB, double>(s, bwd0[i], &shb)));
+        d->shmem_b0 = shb;
         for (int k = 0; k < b.nbv; k++) vmap.push_back(make_int2(i, k));
         sts[i] = s->st;
     }
@@ -2048,6 +2104,9 @@ int batch_build(Solver* const* ss, int n, BatchDev* d) {
     IFL_TRY(upload(s, &d->inst, inst));
     IFL_TRY(upload(s, &d->fwd, fwd));
     IFL_TRY(upload(s, &d->bwd, bwd));
+    IFL_TRY(upload(s, &d->pre, pre));
+    IFL_TRY(upload(s, &d->fwd0, fwd0));
+    IFL_TRY(upload(s, &d->bwd0, bwd0));
     IFL_TRY(upload(s, &d->vmap, vmap));
     IFL_TRY(upload(s, &d->fmap, fmap));
     IFL_TRY(upload(s, &d->bmap, bmap));
@@ -2100,13 +2159,13 @@ int batch_iteration(Solver* s, BatchDev* d, hipStream_t stream, int it, bool pro
     batch_fin_alpha_kernel<<<d->n, 256, 0, stream>>>(d->inst);
     IFL_LAUNCH_CHECK();
     if (prof) IFL_TRY(batch_time(s, d, stream, IFL_KERNEL_SWEEP_FWD, true));
-    batch_sweep_kernel<SW_FWD, LV, NGF><<<d->nsblk, 64 * kWaves, d->shmem_f, stream>>>(d->fwd, d->fmap);
+    batch_sweep_kernel<SW_FWD, 1, LV, NGF><<<d->nsblk, 64 * kWaves, d->shmem_f, stream>>>(d->fwd, d->fmap);
     IFL_LAUNCH_CHECK();
     if (prof) IFL_TRY(batch_time(s, d, stream, IFL_KERNEL_SWEEP_FWD, false));
     batch_fin_check_kernel<<<grid_blocks(d->n, 256), 256, 0, stream>>>(d->inst, d->n);
     IFL_LAUNCH_CHECK();
     if (prof) IFL_TRY(batch_time(s, d, stream, IFL_KERNEL_SWEEP_BWD, true));
-    batch_sweep_kernel<SW_BWD, LV, NGB><<<d->nsblk, 64 * kWaves, d->shmem_b, stream>>>(d->bwd, d->bmap);
+    batch_sweep_kernel<SW_BWD, 1, LV, NGB><<<d->nsblk, 64 * kWaves, d->shmem_b, stream>>>(d->bwd, d->bmap);
     IFL_LAUNCH_CHECK();
     if (prof) IFL_TRY(batch_time(s, d, stream, IFL_KERNEL_SWEEP_BWD, false));
     batch_fin_sigma_kernel<<<d->n, 256, 0, stream>>>(d->inst);
@@ -2173,27 +2232,44 @@ int k_project_batch(Solver* const* ss, int n, hipStream_t stream, BatchDev** dev
                                                                      : batch_build<6, 5, 5>(ss, n, d);
         if (rc != IFL_OK) return rc;
     }
-    // the entry of every solve on its solver's own stream (they overlap),
-    // then the batched iterations on `stream` once every entry is done
+    // the solvers' own stages up to the solve are queued on their streams;
+    // the batch's stream waits for all of them, then builds every factor and
+    // runs every entry of project() (7:854-864) with one launch per phase,
+    // and the batched iterations
     int limit = 0;
+    Solver* s = s0;  // error reporting
     for (int i = 0; i < n; i++) {
-        Solver* s = ss[i];
-        VecArgs a = vec_args(s);
-        long nn = skew_n(s);
-        state_reset_kernel<<<1, 1, 0, s->stream>>>(s->st);
-        IFL_LAUNCH_CHECK();
-        IFL_HIP(hipMemsetAsync(s->p, 0, nn * sizeof(double), s->stream));  // 7:854
-        IFL_TRY(launch_fwd<0>(s));                                         // 7:855
-        IFL_TRY(launch_bwd<0>(s));
-        IFL_HIP(hipMemcpyAsync(s->s, s->z, nn * sizeof(double), hipMemcpyDeviceToDevice, s->stream));  // 7:856
-        fin_sigma_kernel<<<1, 256, 0, s->stream>>>(a, s->sk.nstrips, 0, 1, s->cfg.tol, s->cfg.pcg_limit);
-        IFL_LAUNCH_CHECK();
-        IFL_HIP(hipEventRecord(d->ready[i], s->stream));
+        IFL_HIP(hipEventRecord(d->ready[i], ss[i]->stream));
         IFL_HIP(hipStreamWaitEvent(stream, d->ready[i], 0));
-        limit = std::max(limit, s->cfg.pcg_limit);
+        limit = std::max(limit, ss[i]->cfg.pcg_limit);
     }
-    Solver* s = s0;
     IFL_TRY(batch_tables(s, d, stream, nullptr));  // every instance iterates at first
+    {
+        batch_sweep_kernel<SW_PRECON, 0, 3, 3><<<d->nsblk, 64 * kWaves, d->shmem_p, stream>>>(d->pre, d->fmap);
+        IFL_LAUNCH_CHECK();
+        batch_reset_kernel<<<grid_blocks(n, 256), 256, 0, stream>>>(d->inst, n);
+        IFL_LAUNCH_CHECK();
+        batch_zero_p_kernel<<<d->nvblk, 256, 0, stream>>>(d->inst, d->vmap);  // 7:854
+        IFL_LAUNCH_CHECK();
+        if (lv == 3) {                                                        // 7:855
+            batch_sweep_kernel<SW_FWD, 0, 3, 4><<<d->nsblk, 64 * kWaves, d->shmem_f0, stream>>>(d->fwd0, d->fmap);
+            IFL_LAUNCH_CHECK();
+            batch_sweep_kernel<SW_BWD, 0, 3, 5><<<d->nsblk, 64 * kWaves, d->shmem_b0, stream>>>(d->bwd0, d->bmap);
+        } else if (lv <= 5) {
+            batch_sweep_kernel<SW_FWD, 0, 4, 4><<<d->nsblk, 64 * kWaves, d->shmem_f0, stream>>>(d->fwd0, d->fmap);
+            IFL_LAUNCH_CHECK();
+            batch_sweep_kernel<SW_BWD, 0, 4, 6><<<d->nsblk, 64 * kWaves, d->shmem_b0, stream>>>(d->bwd0, d->bmap);
+        } else {
+            batch_sweep_kernel<SW_FWD, 0, 6, 4><<<d->nsblk, 64 * kWaves, d->shmem_f0, stream>>>(d->fwd0, d->fmap);
+            IFL_LAUNCH_CHECK();
+            batch_sweep_kernel<SW_BWD, 0, 6, 5><<<d->nsblk, 64 * kWaves, d->shmem_b0, stream>>>(d->bwd0, d->bmap);
+        }
+        IFL_LAUNCH_CHECK();
+        batch_copy_s_kernel<<<d->nvblk, 256, 0, stream>>>(d->inst, d->vmap);  // 7:856
+        IFL_LAUNCH_CHECK();
+        batch_fin_sigma_init_kernel<<<n, 256, 0, stream>>>(d->inst);           // 7:858-864
+        IFL_LAUNCH_CHECK();
+    }
     int launched = 0, batch = 4, iterating = n;
     std::vector<char> active(n, 1);
     while (true) {
