@@ -69,3 +69,38 @@ def test_solver7_2048_first_step():
     # iterations of a nearly singular system the count may move slightly
     assert abs(got[1] - iters[1]) <= max(2, iters[1] // 100)
     gpu.close()
+
+
+def test_solver7_1024_tree_against_sequential():
+    """The timed form (tree-ordered dot products) against the bit-exact form
+    (the reference's sequential sums, pinned to the reference build at 1024^2
+    by test_gpu_configs.py), one step of the benchmark scene at 1024x1024
+    from the same state: the fields differ only by the rounding of the dot
+    products. Printed per field; held to 1e-6 of each field's largest
+    magnitude (p less its FLUID mean), with the same heat iteration count and
+    the pressure count within 1 %."""
+    import ifl_fluids as F
+    import ifl_scenes as S
+    n = 1024
+    sc = S.config3_scene()
+    objs = [F.SolidBox(*b[1:]) for b in sc["bodies"]]
+    res = {}
+    for order in (F.DOT_TREE, F.DOT_SEQUENTIAL):
+        sv = F.FluidSolver(7, n, n, sc["params"], objs, dot_order=order)
+        for inf in sc["inflows"]:
+            sv.add_inflow(*inf)
+        st = sv.update(sc["dt"])
+        assert st.pressure.status == F.SOLVE_CONVERGED and st.heat.status == F.SOLVE_CONVERGED
+        fl = sv.field(F.Q_D * 16 + F.QF_CELL) == 0
+        vals = {k: sv.field(f) for k, f in (("d", F.Q_D * 16), ("t", F.Q_T * 16), ("u", F.Q_U * 16),
+                                             ("v", F.Q_V * 16), ("p", F.F_P))}
+        vals["p"] = np.where(fl, vals["p"] - vals["p"][fl].mean(), 0.0)
+        res[order] = (vals, st.heat.iterations, st.pressure.iterations)
+        sv.close()
+    (a, ha, pa), (b, hb, pb) = res[F.DOT_TREE], res[F.DOT_SEQUENTIAL]
+    print(f"1024x1024 iterations heat {ha}/{hb}, pressure {pa}/{pb} (tree/sequential)", flush=True)
+    assert ha == hb and abs(pa - pb) <= 0.01 * pb
+    for k in a:
+        err = np.abs(a[k] - b[k]).max() / max(np.abs(b[k]).max(), 1e-300)
+        print(f"  {k}: tree vs sequential {err:.2e}", flush=True)
+        assert err <= 1e-6, k
