@@ -23,6 +23,7 @@ struct ifl_batch {
     hipStream_t stream = nullptr;
     int device = 0;
     ifl::BatchDev* dev = nullptr;
+    ifl::ExtrapDev* xdev = nullptr;
     bool profiling = false;
     ifl::BatchProfile prof;
 };
@@ -447,7 +448,7 @@ int update_part(Solver* s, int after, double dt, int* next) {
         IFL_TRY(extrapolate(s, IFL_Q_T));
         IFL_TRY(k_add_buoyancy(s, dt));
     }
-    if (after != SOLVE_PRESSURE) {
+    if (after != SOLVE_PRESSURE && after != STAGE_EXTRAPOLATE) {
         IFL_TRY(k_set_boundary(s));
         IFL_TRY(k_build_rhs(s));
         if (lv >= 7) IFL_TRY(k_compute_densities(s));
@@ -456,17 +457,23 @@ int update_part(Solver* s, int after, double dt, int* next) {
         *next = SOLVE_PRESSURE;
         return IFL_OK;
     }
-    IFL_TRY(k_apply_pressure(s, dt));
-    if (lv == 3) {
-        IFL_TRY(k_advect(s, (1 << IFL_Q_D) | (1 << IFL_Q_U) | (1 << IFL_Q_V), dt));
-        flip_q(s, IFL_Q_D);
-        flip_q(s, IFL_Q_U);
-        flip_q(s, IFL_Q_V);
-        return IFL_OK;
+    if (after == SOLVE_PRESSURE) {
+        IFL_TRY(k_apply_pressure(s, dt));
+        if (lv == 3) {
+            IFL_TRY(k_advect(s, (1 << IFL_Q_D) | (1 << IFL_Q_U) | (1 << IFL_Q_V), dt));
+            flip_q(s, IFL_Q_D);
+            flip_q(s, IFL_Q_U);
+            flip_q(s, IFL_Q_V);
+            return IFL_OK;
+        }
+        if (s->batched && lv < 8) {
+            *next = STAGE_EXTRAPOLATE;  // k_extrapolate_batch
+            return IFL_OK;
+        }
+        IFL_TRY(extrapolate(s, IFL_Q_D));
+        IFL_TRY(extrapolate(s, IFL_Q_U));
+        IFL_TRY(extrapolate(s, IFL_Q_V));
     }
-    IFL_TRY(extrapolate(s, IFL_Q_D));
-    IFL_TRY(extrapolate(s, IFL_Q_U));
-    IFL_TRY(extrapolate(s, IFL_Q_V));
     if (lv < 8) IFL_TRY(defer_border_flag(s));
     IFL_TRY(k_set_boundary(s));
     if (lv == 8) {
@@ -967,8 +974,11 @@ int ifl_batch_update(ifl_batch* b, double dt, ifl_step_stats* stats) {
             next = ni;  // every solver of one level takes the same sequence of solves
         }
         if (rc != IFL_OK || next == SOLVE_NONE) break;
-        rc = k_project_batch(b->solvers.data(), n, b->stream, &b->dev,
-                             next == SOLVE_HEAT ? heat.data() : press.data(), b->profiling ? &b->prof : nullptr);
+        if (next == STAGE_EXTRAPOLATE)
+            rc = k_extrapolate_batch(b->solvers.data(), n, b->stream, &b->xdev);
+        else
+            rc = k_project_batch(b->solvers.data(), n, b->stream, &b->dev,
+                                 next == SOLVE_HEAT ? heat.data() : press.data(), b->profiling ? &b->prof : nullptr);
         if (rc != IFL_OK) g_err = b->solvers[0]->err;
         after = next;
     }
@@ -1003,6 +1013,7 @@ int ifl_batch_destroy(ifl_batch* b) {
         (void)hipStreamDestroy(b->stream);
     }
     batch_dev_free(b->dev);
+    extrap_dev_free(b->xdev);
     delete b;
     return IFL_OK;
 }

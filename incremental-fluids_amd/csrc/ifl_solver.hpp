@@ -202,7 +202,8 @@ int k_skew_to_rows(Solver* s, const double* skew, double* rows);
 int k_rows_to_skew(Solver* s, const double* rows, double* skew);
 
 // the linear solves of one update() (FluidSolver::project calls)
-enum { SOLVE_NONE = 0, SOLVE_HEAT = 1, SOLVE_PRESSURE = 2 };
+enum { SOLVE_NONE = 0, SOLVE_HEAT = 1, SOLVE_PRESSURE = 2,
+       STAGE_EXTRAPOLATE = 3 };  // batched solvers (4-7): d, u, v extrapolated by the batch
 
 // ---- launchers (k_pcg.hip) ----
 int k_build_precon(Solver* s);
@@ -215,6 +216,11 @@ int k_project(Solver* s, int limit, ifl_solve_stats* stats);
 // tail on its own stream, the iterations on `stream`, one launch per phase of
 // each PCG iteration for all of them (stats: n entries)
 struct BatchDev;
+struct ExtrapDev;
+// the batched form's extrapolation of d, u and v (extrapolate 7:560-588) for
+// solvers of levels 4-7, one launch per phase for all of them
+int k_extrapolate_batch(Solver* const* ss, int n, hipStream_t stream, ExtrapDev** dev);
+void extrap_dev_free(ExtrapDev* d);
 // per-kernel totals of a batch's fused PCG launches (when profiling): every
 // launch timed with events on the batch's stream; cell_iterations counts the
 // cells of the instances that did work in them

@@ -164,9 +164,8 @@ int k_fill_solid(Solver* s, int qi) {
 // ------------------------------------------------------------ extrapolate
 // fillSolidMask 7:524-542: interior non-fluid cells record which upstream
 // neighbours (along the normal) are still unknown.
-__global__ void solid_mask_kernel(QtyView q, int* __restrict__ list, int* __restrict__ cnt) {
+IFL_DEV void solid_mask_body(const QtyView& q, int* __restrict__ list, int* __restrict__ cnt, int i) {
     int w = q.w, h = q.h;
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= w * h) return;
     int x = i % w, y = i / w;
     if (x < 1 || y < 1 || x >= w - 1 || y >= h - 1) return;
@@ -177,6 +176,9 @@ __global__ void solid_mask_kernel(QtyView q, int* __restrict__ list, int* __rest
     if (ny != 0.0 && q.cell[i + sgn(ny) * w] != IFL_CELL_FLUID) m |= 2;
     q.mask[i] = m;
     if (m == 0) list[atomicAdd(cnt, 1)] = i;  // initial ready set, 7:563-571
+}
+__global__ void solid_mask_kernel(QtyView q, int* __restrict__ list, int* __restrict__ cnt) {
+    solid_mask_body(q, list, cnt, blockIdx.x * blockDim.x + threadIdx.x);
 }
 
 IFL_DEV double src_safe(const QtyView& q, long i) { return (i >= 0 && i < (long)q.w * q.h) ? q.src[i] : 0.0; }
@@ -193,8 +195,8 @@ IFL_DEV double extrap_normal(const QtyView& q, int i) {
 // becomes ready exactly once, when its last upstream neighbour is final, and
 // its value reads only upstream values, so processing ready cells level by
 // level gives the same values in any order. One workgroup walks the levels.
-__global__ __launch_bounds__(1024) void extrapolate_levels_kernel(QtyView q, int* __restrict__ lists, int* __restrict__ cnt, long cap) {
-    __shared__ int s_next;
+IFL_DEV void extrapolate_levels_body(const QtyView& q, int* __restrict__ lists, int* __restrict__ cnt, long cap,
+                                     int& s_next) {
     int w = q.w;
     long n = (long)q.w * q.h;
     int* cur = lists;
@@ -240,9 +242,12 @@ __global__ __launch_bounds__(1024) void extrapolate_levels_kernel(QtyView q, int
         __syncthreads();
     }
 }
+__global__ __launch_bounds__(1024) void extrapolate_levels_kernel(QtyView q, int* __restrict__ lists, int* __restrict__ cnt, long cap) {
+    __shared__ int s_next;
+    extrapolate_levels_body(q, lists, cnt, cap, s_next);
+}
 
-__global__ void border_solid_kernel(QtyView q, int* flag) {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
+IFL_DEV void border_solid_body(const QtyView& q, int* flag, int i) {
     int w = q.w, h = q.h;
     int perim = 2 * w + 2 * h;
     if (i >= perim) return;
@@ -252,6 +257,9 @@ __global__ void border_solid_kernel(QtyView q, int* flag) {
     else if (i < 2 * w + h) { x = 0; y = i - 2 * w; }
     else { x = w - 1; y = i - 2 * w - h; }
     if (q.cell[x + y * w] != IFL_CELL_FLUID) atomicOr(flag, 1);
+}
+__global__ void border_solid_kernel(QtyView q, int* flag) {
+    border_solid_body(q, flag, blockIdx.x * blockDim.x + threadIdx.x);
 }
 
 int k_extrapolate(Solver* s, int qi) {
@@ -267,6 +275,121 @@ int k_extrapolate(Solver* s, int qi) {
     long cap = (long)(s->W + 1) * (s->H + 1);
     extrapolate_levels_kernel<<<1, 1024, 0, s->stream>>>(q, s->front, s->front_cnt, cap);
     IFL_LAUNCH_CHECK();
+    return IFL_OK;
+}
+
+// ------------------------------------------------------------ batched extrapolation
+// The batched form's extrapolations of d, u and v: for each quantity, one
+// launch per phase (counters, fillSolidMask, the border flag, the level walk
+// with one workgroup per solver) over a table of every solver's quantity,
+// the single-solver kernels' bodies.
+struct ExtrapInst {
+    QtyView q;
+    int* front;
+    int* front_cnt;
+    long cap;
+};
+
+__global__ void batch_extrap_zero_kernel(const ExtrapInst* t, int n) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    t[i].front_cnt[0] = 0;
+    t[i].front_cnt[1] = 0;
+}
+__global__ __launch_bounds__(256) void batch_solid_mask_kernel(const ExtrapInst* t, const int2* map) {
+    const int2 m = map[blockIdx.x];
+    solid_mask_body(t[m.x].q, t[m.x].front, t[m.x].front_cnt, m.y * 256 + threadIdx.x);
+}
+__global__ __launch_bounds__(256) void batch_border_solid_kernel(const ExtrapInst* t, const int2* map) {
+    const int2 m = map[blockIdx.x];
+    border_solid_body(t[m.x].q, t[m.x].front_cnt + 1, m.y * 256 + threadIdx.x);
+}
+__global__ __launch_bounds__(1024) void batch_extrapolate_levels_kernel(const ExtrapInst* t) {
+    __shared__ int s_next;
+    const ExtrapInst& e = t[blockIdx.x];
+    extrapolate_levels_body(e.q, e.front, e.front_cnt, e.cap, s_next);
+}
+
+struct ExtrapDev {
+    std::vector<Solver*> solvers;
+    ExtrapInst* inst_h[3] = {nullptr, nullptr, nullptr};  // pinned, one table per quantity
+    ExtrapInst* inst_d[3] = {nullptr, nullptr, nullptr};
+    int2* mask_map[3] = {nullptr, nullptr, nullptr};
+    int2* border_map[3] = {nullptr, nullptr, nullptr};
+    int nmask[3] = {0, 0, 0}, nborder[3] = {0, 0, 0};
+    std::vector<hipEvent_t> ready;
+    hipEvent_t joined = nullptr;
+};
+
+void extrap_dev_free(ExtrapDev* d) {
+    if (!d) return;
+    for (int k = 0; k < 3; k++) {
+        if (d->inst_h[k]) (void)hipHostFree(d->inst_h[k]);
+        if (d->inst_d[k]) (void)hipFree(d->inst_d[k]);
+        if (d->mask_map[k]) (void)hipFree(d->mask_map[k]);
+        if (d->border_map[k]) (void)hipFree(d->border_map[k]);
+    }
+    for (hipEvent_t e : d->ready) (void)hipEventDestroy(e);
+    if (d->joined) (void)hipEventDestroy(d->joined);
+    delete d;
+}
+
+int k_extrapolate_batch(Solver* const* ss, int n, hipStream_t stream, ExtrapDev** dev) {
+    static const int kQ[3] = {IFL_Q_D, IFL_Q_U, IFL_Q_V};
+    Solver* s = ss[0];  // error reporting
+    ExtrapDev* d = *dev;
+    if (!d || d->solvers.size() != (size_t)n || !std::equal(ss, ss + n, d->solvers.begin())) {
+        extrap_dev_free(d);
+        d = *dev = new ExtrapDev();
+        d->solvers.assign(ss, ss + n);
+        for (int k = 0; k < 3; k++) {
+            std::vector<int2> mm, bm;
+            for (int i = 0; i < n; i++) {
+                const QtyView& q = ss[i]->q[kQ[k]];
+                const long cells = (long)q.w * q.h, perim = 2L * q.w + 2L * q.h;
+                for (long b = 0; b < (cells + 255) / 256; b++) mm.push_back(make_int2(i, (int)b));
+                for (long b = 0; b < (perim + 255) / 256; b++) bm.push_back(make_int2(i, (int)b));
+            }
+            d->nmask[k] = (int)mm.size();
+            d->nborder[k] = (int)bm.size();
+            IFL_HIP(hipMalloc((void**)&d->mask_map[k], mm.size() * sizeof(int2)));
+            IFL_HIP(hipMemcpy(d->mask_map[k], mm.data(), mm.size() * sizeof(int2), hipMemcpyHostToDevice));
+            IFL_HIP(hipMalloc((void**)&d->border_map[k], bm.size() * sizeof(int2)));
+            IFL_HIP(hipMemcpy(d->border_map[k], bm.data(), bm.size() * sizeof(int2), hipMemcpyHostToDevice));
+            IFL_HIP(hipHostMalloc((void**)&d->inst_h[k], n * sizeof(ExtrapInst), hipHostMallocDefault));
+            IFL_HIP(hipMalloc((void**)&d->inst_d[k], n * sizeof(ExtrapInst)));
+        }
+        d->ready.assign(n, nullptr);
+        for (int i = 0; i < n; i++) IFL_HIP(hipEventCreateWithFlags(&d->ready[i], hipEventDisableTiming));
+        IFL_HIP(hipEventCreateWithFlags(&d->joined, hipEventDisableTiming));
+    }
+    // the solvers' src arrays swap every step, so the tables are written
+    // afresh (the previous step's copies completed before it returned)
+    for (int k = 0; k < 3; k++)
+        for (int i = 0; i < n; i++) {
+            ExtrapInst& e = d->inst_h[k][i];
+            e.q = ss[i]->q[kQ[k]];
+            e.front = ss[i]->front;
+            e.front_cnt = ss[i]->front_cnt;
+            e.cap = (long)(ss[i]->W + 1) * (ss[i]->H + 1);
+        }
+    for (int i = 0; i < n; i++) {
+        IFL_HIP(hipEventRecord(d->ready[i], ss[i]->stream));
+        IFL_HIP(hipStreamWaitEvent(stream, d->ready[i], 0));
+    }
+    for (int k = 0; k < 3; k++) {
+        IFL_HIP(hipMemcpyAsync(d->inst_d[k], d->inst_h[k], n * sizeof(ExtrapInst), hipMemcpyHostToDevice, stream));
+        batch_extrap_zero_kernel<<<grid_blocks(n, 256), 256, 0, stream>>>(d->inst_d[k], n);
+        IFL_LAUNCH_CHECK();
+        batch_solid_mask_kernel<<<d->nmask[k], 256, 0, stream>>>(d->inst_d[k], d->mask_map[k]);
+        IFL_LAUNCH_CHECK();
+        batch_border_solid_kernel<<<d->nborder[k], 256, 0, stream>>>(d->inst_d[k], d->border_map[k]);
+        IFL_LAUNCH_CHECK();
+        batch_extrapolate_levels_kernel<<<n, 1024, 0, stream>>>(d->inst_d[k]);
+        IFL_LAUNCH_CHECK();
+    }
+    IFL_HIP(hipEventRecord(d->joined, stream));
+    for (int i = 0; i < n; i++) IFL_HIP(hipStreamWaitEvent(ss[i]->stream, d->joined, 0));
     return IFL_OK;
 }
 
