@@ -276,19 +276,23 @@ def mixed_frame_errors(n, frames=2, warm=2, form="mixed"):
 #     grids read (32 B/N)
 #   particle advect: posx, posy read and written (32 B/P); u, v read (16 B/N)
 #   P2G, the reduced-precision forms' LDS-binned scatter, counted as the
-#     transfer: posx, posy and the properties read once (d with t 32 B/P,
-#     u or v 24 B/P); src (and src t), weight and the cell bytes written
-#     (26 B/N, 17 B/N). The binned records are the kernels' own traffic.
+#     transfer: posx, posy and the properties read once (d with t 16 B/P,
+#     u or v 12 B/P, float particles); src (and src t), weight and the cell
+#     bytes written (26 B/N, 17 B/N). The binned records are the kernels'
+#     own traffic.
+# The reduced-precision forms keep the particle state in float: G2P 40 B/P,
+# the particle advection 16 B/P.
 def flip_kernel_bytes(n, p_before, p_after, binned=False):
     nd, nu, nv = n * n, (n + 1) * n, n * (n + 1)
+    pb = 4 if binned else 8   # bytes per particle value
     if binned:
-        p2g = (32 * p_before + 26 * nd) + (24 * p_before + 17 * nu) + (24 * p_before + 17 * nv)
+        p2g = (4 * pb * p_before + 26 * nd) + (3 * pb * p_before + 17 * nu) + (3 * pb * p_before + 17 * nv)
     else:
         p2g = (36 * p_before + 30 * nd) + (28 * p_before + 21 * nu) + (28 * p_before + 21 * nv)
     return {
         "p2g": p2g,
-        "g2p": 80 * p_after + 32 * nd,
-        "particle_advect": 32 * p_after + 16 * nd,
+        "g2p": 10 * pb * p_after + 32 * nd,
+        "particle_advect": 4 * pb * p_after + 16 * nd,
     }
 
 

@@ -188,7 +188,9 @@ enum {
     IFL_FIELD_PROP_V = 85, IFL_FIELD_WEIGHT = 86, IFL_FIELD_COUNTS = 87
 };
 
-/* Element count and size (8, 4 or 1 bytes) of a field; 0 count if absent. */
+/* Element count and size (8, 4 or 1 bytes) of a field; 0 count if absent.
+ * IFL_FIELD_POSX..IFL_FIELD_PROP_V are doubles in IFL_PREC_FP64 and floats
+ * (4 bytes) in the reduced-precision forms; IFL_FIELD_COUNTS is int32. */
 int ifl_field_info(const ifl_solver* s, int field, size_t* count, int* elem_bytes);
 int ifl_upload(ifl_solver* s, int field, const void* host, size_t bytes);
 int ifl_download(ifl_solver* s, int field, void* host, size_t bytes);

@@ -164,6 +164,12 @@ int dalloc(Solver* s, double** p, size_t n) {
     IFL_HIP(hipMemsetAsync(*p, 0, n * sizeof(double), s->stream));
     return IFL_OK;
 }
+int falloc(Solver* s, float** p, size_t n) {
+    n = n ? n : 1;
+    IFL_HIP(hipMalloc((void**)p, n * sizeof(float)));
+    IFL_HIP(hipMemsetAsync(*p, 0, n * sizeof(float), s->stream));
+    return IFL_OK;
+}
 int balloc(Solver* s, uint8_t** p, size_t n) {
     n = (n + 7) & ~(size_t)7;  // whole 32-bit words for the mask atomics
     IFL_HIP(hipMalloc((void**)p, n));
@@ -224,11 +230,16 @@ void* field_ptr(Solver* s, int f, size_t* count, int* elem) {
     }
     if (lv != 8) return nullptr;
     switch (f) {
-    case IFL_FIELD_POSX: *count = s->max_particles; return s->posx;
-    case IFL_FIELD_POSY: *count = s->max_particles; return s->posy;
-    case IFL_FIELD_PROP_D: case IFL_FIELD_PROP_T: case IFL_FIELD_PROP_U: case IFL_FIELD_PROP_V:
+    case IFL_FIELD_POSX: case IFL_FIELD_POSY:
+    case IFL_FIELD_PROP_D: case IFL_FIELD_PROP_T: case IFL_FIELD_PROP_U: case IFL_FIELD_PROP_V: {
         *count = s->max_particles;
-        return s->prop[f - IFL_FIELD_PROP_D];
+        const int k = f == IFL_FIELD_POSX ? 0 : f == IFL_FIELD_POSY ? 1 : 2 + (f - IFL_FIELD_PROP_D);
+        if (s->fposx) {  // float particle state
+            *elem = 4;
+            return k == 0 ? s->fposx : k == 1 ? s->fposy : s->fprop[k - 2];
+        }
+        return k == 0 ? s->posx : k == 1 ? s->posy : s->prop[k - 2];
+    }
     case IFL_FIELD_WEIGHT: *count = (size_t)(s->W + 1) * (s->H + 1); return s->weight;
     case IFL_FIELD_COUNTS: *count = n; *elem = 4; return s->counts;
     }
@@ -247,8 +258,8 @@ void free_all(Solver* s) {
     F(s->cx32); F(s->cy32); F(s->pc32); F(s->y32); F(s->partial); F(s->st);
     F(s->e32); F(s->r32); F(s->s32); F(s->z32); F(s->q32); F(s->ad32); F(s->apx32); F(s->apy32); F(s->st_outer);
     F(s->stage_buf); F(s->front); F(s->front_cnt); F(s->bodies_d); F(s->hreq); F(s->hreq_cnt);
-    F(s->posx); F(s->posy);
-    for (int t = 0; t < 4; t++) F(s->prop[t]);
+    F(s->posx); F(s->posy); F(s->fposx); F(s->fposy);
+    for (int t = 0; t < 4; t++) { F(s->prop[t]); F(s->fprop[t]); }
     F(s->weight); F(s->counts); F(s->image_d); F(s->step_flags);
     if (s->step_flags_host) (void)hipHostFree(s->step_flags_host);
     flip_free(s);
@@ -356,9 +367,15 @@ int init_solver(Solver* s) {
         // property arrays start at zero (8:896-897)
         s->max_particles = W * H * 12;
         size_t mp = (size_t)s->max_particles;
-        IFL_TRY(dalloc(s, &s->posx, mp));
-        IFL_TRY(dalloc(s, &s->posy, mp));
-        for (int t = 0; t < 4; t++) IFL_TRY(dalloc(s, &s->prop[t], mp));
+        if (s->cfg.precision == IFL_PREC_FP64) {
+            IFL_TRY(dalloc(s, &s->posx, mp));
+            IFL_TRY(dalloc(s, &s->posy, mp));
+            for (int t = 0; t < 4; t++) IFL_TRY(dalloc(s, &s->prop[t], mp));
+        } else {  // the reduced-precision forms keep the particles in float
+            IFL_TRY(falloc(s, &s->fposx, mp));
+            IFL_TRY(falloc(s, &s->fposy, mp));
+            for (int t = 0; t < 4; t++) IFL_TRY(falloc(s, &s->fprop[t], mp));
+        }
         IFL_TRY(dalloc(s, &s->weight, (size_t)(W + 1) * (H + 1)));
         IFL_HIP(hipMalloc((void**)&s->counts, (size_t)W * H * sizeof(int)));
         IFL_HIP(hipMemsetAsync(s->counts, 0, (size_t)W * H * sizeof(int), s->stream));

@@ -102,6 +102,8 @@ struct Solver {
     int max_particles = 0;
     int particle_count = 0;
     double *posx = nullptr, *posy = nullptr, *prop[4] = {nullptr, nullptr, nullptr, nullptr};
+    // the particle state of the reduced-precision forms (in place of the doubles above)
+    float *fposx = nullptr, *fposy = nullptr, *fprop[4] = {nullptr, nullptr, nullptr, nullptr};
     double* weight = nullptr;
     int* counts = nullptr;
     bool particles_ready = false;  // initParticles has run

@@ -86,8 +86,11 @@ __global__ void init_count_kernel(int W, int H, uint32_t s0, const Body* __restr
     bits[c] = b;
 }
 
+// P: the particle state's element type (double, or float in the
+// reduced-precision forms; a jittered position is a float sum, exact in both)
+template <typename P>
 __global__ void init_write_kernel(int W, int H, uint32_t s0, const int* __restrict__ cnt, const int* __restrict__ off,
-                                  const uint8_t* __restrict__ bits, double* __restrict__ posx, double* __restrict__ posy,
+                                  const uint8_t* __restrict__ bits, P* __restrict__ posx, P* __restrict__ posy,
                                   FlipScalars* sc) {
     long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
     long n = (long)W * H;
@@ -103,13 +106,13 @@ __global__ void init_write_kernel(int W, int H, uint32_t s0, const int* __restri
         double py = jitter(y, s);
         bool last = (c == n - 1) && (i == kAvgPerCell - 1);
         if (b & (1u << i)) {
-            posx[j] = px;
-            posy[j] = py;
+            posx[j] = (P)px;
+            posy[j] = (P)py;
             j++;
         } else if (last) {
             // a rejected draw stays in the slot past the last particle (8:743-748)
-            posx[j] = px;
-            posy[j] = py;
+            posx[j] = (P)px;
+            posy[j] = (P)py;
         }
     }
     if (c == n - 1) {
@@ -119,16 +122,18 @@ __global__ void init_write_kernel(int W, int H, uint32_t s0, const int* __restri
 }
 
 // ------------------------------------------------------------ gridToParticles 8:904-911
+template <typename P>
 struct G2PArgs {
     QtyView q[4];
-    const double* posx;
-    const double* posy;
-    double* prop[4];
+    const P* posx;
+    const P* posy;
+    P* prop[4];
     int n;
     double alpha;
 };
 
-__global__ void g2p_kernel(G2PArgs a) {
+template <typename P>
+__global__ void g2p_kernel(G2PArgs<P> a) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= a.n) return;
     double x = a.posx[i], y = a.posy[i];
@@ -138,7 +143,7 @@ __global__ void g2p_kernel(G2PArgs a) {
         double p = a.prop[t][i];
         p *= 1.0 - a.alpha;
         p += qlerp(q.src, q.w, q.h, q.ox, q.oy, x, y);
-        a.prop[t][i] = p;
+        a.prop[t][i] = (P)p;
     }
 }
 
@@ -156,7 +161,8 @@ __global__ void diff_kernel(double* __restrict__ src, const double* __restrict__
 }
 
 // ------------------------------------------------------------ countParticles 8:754-763
-__global__ void count_kernel(const double* __restrict__ posx, const double* __restrict__ posy, int n, int W, int H,
+template <typename P>
+__global__ void count_kernel(const P* __restrict__ posx, const P* __restrict__ posy, int n, int W, int H,
                              int* __restrict__ counts) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
@@ -169,7 +175,8 @@ __global__ void count_kernel(const double* __restrict__ posx, const double* __re
 // A particle can only be removed while its cell holds more than 12, and
 // counts only fall, so the candidates are the particles of cells that start
 // above 12.
-__global__ void prune_flag_kernel(const double* __restrict__ posx, const double* __restrict__ posy, int n, int W, int H,
+template <typename P>
+__global__ void prune_flag_kernel(const P* __restrict__ posx, const P* __restrict__ posy, int n, int W, int H,
                                   const int* __restrict__ counts, uint8_t* __restrict__ flag) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
@@ -183,7 +190,8 @@ __global__ void prune_flag_kernel(const double* __restrict__ posx, const double*
 // shrinking end are only ever read (each is the "last" particle once), so the
 // particle found at a slot is the original one until the slot is visited;
 // the swaps are recorded and applied afterwards in parallel.
-__global__ void prune_walk_kernel(const double* __restrict__ posx, const double* __restrict__ posy, int n, int W, int H,
+template <typename P>
+__global__ void prune_walk_kernel(const P* __restrict__ posx, const P* __restrict__ posy, int n, int W, int H,
                                   int* __restrict__ counts, const int* __restrict__ cand, const int* __restrict__ ncand,
                                   int* __restrict__ mv_dst, int* __restrict__ mv_src, FlipScalars* sc) {
     if (threadIdx.x != 0) return;
@@ -216,14 +224,16 @@ __global__ void prune_walk_kernel(const double* __restrict__ posx, const double*
     sc->moves = nm;
 }
 
+template <typename P>
 struct ParticleArrays {
-    double* a[6];  // posx, posy, prop d, t, u, v
+    P* a[6];  // posx, posy, prop d, t, u, v
 };
 
 // Sources lie at or past the final end and destinations below it (or at it,
 // for a particle moved in and then dropped), so the copies are independent;
 // repeated moves into one slot are consecutive and only the last one counts.
-__global__ void prune_apply_kernel(ParticleArrays pa, const int* __restrict__ mv_dst, const int* __restrict__ mv_src,
+template <typename P>
+__global__ void prune_apply_kernel(ParticleArrays<P> pa, const int* __restrict__ mv_dst, const int* __restrict__ mv_src,
                                    const FlipScalars* sc) {
     int nm = sc->moves;
     for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < nm; k += gridDim.x * blockDim.x) {
@@ -247,7 +257,8 @@ __global__ void seed_attempts_kernel(const int* __restrict__ counts, long n, int
 // the new particle. When every cell index is below the count at the start of
 // the pass, that particle is never written by the pass, so a cell accepts or
 // rejects all of its attempts at once.
-__global__ void seed_accept_kernel(const double* __restrict__ posx, const double* __restrict__ posy,
+template <typename P>
+__global__ void seed_accept_kernel(const P* __restrict__ posx, const P* __restrict__ posy,
                                    const int* __restrict__ att, long n, const Body* __restrict__ bodies, int nb,
                                    double hx, const FlipScalars* sc, int* __restrict__ acc) {
     long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -273,9 +284,10 @@ __global__ void seed_plan_kernel(const int* __restrict__ att, const int* __restr
         sc->executed = aoff[i] + (mfree - 1 - joff[i]) + 1;
 }
 
+template <typename P>
 struct SeedArgs {
     QtyView q[4];
-    ParticleArrays pa;
+    ParticleArrays<P> pa;
     const int* att;
     const int* aoff;
     const int* acc;
@@ -287,15 +299,17 @@ struct SeedArgs {
     FlipScalars* sc;
 };
 
-IFL_DEV void seed_props(const SeedArgs& a, int j, double px, double py) {
+template <typename P>
+IFL_DEV void seed_props(const SeedArgs<P>& a, int j, double px, double py) {
 #pragma unroll
     for (int t = 0; t < 4; t++) {
         const QtyView& q = a.q[t];
-        a.pa.a[2 + t][j] = qlerp(q.src, q.w, q.h, q.ox, q.oy, px, py);
+        a.pa.a[2 + t][j] = (P)qlerp(q.src, q.w, q.h, q.ox, q.oy, px, py);
     }
 }
 
-__global__ void seed_write_kernel(SeedArgs a) {
+template <typename P>
+__global__ void seed_write_kernel(SeedArgs<P> a) {
     long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= a.n) return;
     int p0 = a.sc->count;
@@ -316,15 +330,15 @@ __global__ void seed_write_kernel(SeedArgs a) {
         double py = jitter(y, s);
         if (accept) {
             int j = p0 + a.joff[i] + r;
-            a.pa.a[0][j] = px;
-            a.pa.a[1][j] = py;
+            a.pa.a[0][j] = (P)px;
+            a.pa.a[1][j] = (P)py;
             seed_props(a, j, px, py);
         } else if (at == executed - 1) {
             // the last draw was rejected: it stays in the slot past the last
             // particle (8:800-803)
             int j = p0 + a.joff[a.n - 1] + a.acc[a.n - 1];
-            a.pa.a[0][j] = px;
-            a.pa.a[1][j] = py;
+            a.pa.a[0][j] = (P)px;
+            a.pa.a[1][j] = (P)py;
         }
     }
 }
@@ -340,7 +354,8 @@ __global__ void seed_finish_kernel(const int* __restrict__ acc, const int* __res
 
 // Fewer particles than cells: particle `idx` may be one this pass writes, so
 // the pass is replayed in order by one thread.
-__global__ void seed_sequential_kernel(SeedArgs a, const int* __restrict__ counts, const Body* __restrict__ bodies,
+template <typename P>
+__global__ void seed_sequential_kernel(SeedArgs<P> a, const int* __restrict__ counts, const Body* __restrict__ bodies,
                                        int nb, double hx) {
     if (threadIdx.x != 0) return;
     int cnt = a.sc->count;
@@ -352,9 +367,9 @@ __global__ void seed_sequential_kernel(SeedArgs a, const int* __restrict__ count
             if (cnt == a.max_particles) goto done;
             int j = cnt;
             s = lcg_next(s);
-            a.pa.a[0][j] = jitter(x, s);
+            a.pa.a[0][j] = (P)jitter(x, s);
             s = lcg_next(s);
-            a.pa.a[1][j] = jitter(y, s);
+            a.pa.a[1][j] = (P)jitter(y, s);
             if (point_in_body(bodies, nb, hx, a.pa.a[0][idx], a.pa.a[1][idx])) continue;
             seed_props(a, j, a.pa.a[0][j], a.pa.a[1][j]);
             cnt++;
@@ -483,14 +498,14 @@ struct P2G32Args {
     int n;                   // particles
     int nq;                  // grids transferred (1, or 3 for particlesToGrid)
     QtyView q[3];
-    const double* prop[3];
-    const double* prop2;     // temperature with density (grid 0), or null
+    const float* prop[3];
+    const float* prop2;      // temperature with density (grid 0), or null
     uint8_t* cell2;
     double* src2;
     int ntx[3];              // tiles per row of each grid
     int tile_base[4];        // grid g's tiles are [tile_base[g], tile_base[g + 1])
-    const double* posx;
-    const double* posy;
+    const float* posx;
+    const float* posy;
     int* tile_cnt;           // all tiles
     int* tile_start;         // all tiles + 1
     int* tile_fill;          // all tiles
@@ -556,8 +571,8 @@ __global__ void p2g32_scatter_kernel(P2G32Args a) {
         const int k = a.tile_start[key] + slot;
         const float fx = fminf((float)(x - ix), below1), fy = fminf((float)(y - iy), below1);
         const int local = (ix % kP2GTile) | ((iy % kP2GTile) << 8);
-        a.rec[k] = make_float4(__int_as_float(local), fx, fy, (float)a.prop[g][i]);
-        if (g == 0 && a.prop2) a.rec2[k] = (float)a.prop2[i];
+        a.rec[k] = make_float4(__int_as_float(local), fx, fy, a.prop[g][i]);
+        if (g == 0 && a.prop2) a.rec2[k] = a.prop2[i];
     }
 }
 
@@ -907,10 +922,11 @@ __global__ void empty_to_fluid_kernel(uint8_t* __restrict__ cell, long n) {
 }
 
 // ------------------------------------------------------------ ParticleQuantities::advect 8:931-939
+template <typename P>
 struct AdvectPArgs {
     QtyView u, v;
-    double* posx;
-    double* posy;
+    P* posx;
+    P* posy;
     int n, W, H;
     double hx, dt;
     const Body* bodies;
@@ -918,7 +934,8 @@ struct AdvectPArgs {
 };
 
 // backProject 8:816-839
-IFL_DEV void particle_back_project(const AdvectPArgs& a, double& x, double& y) {
+template <typename P>
+IFL_DEV void particle_back_project(const AdvectPArgs<P>& a, double& x, double& y) {
     double d = 1e30;
     int closest = -1;
     for (int i = 0; i < a.nb; i++) {
@@ -943,7 +960,8 @@ IFL_DEV void particle_back_project(const AdvectPArgs& a, double& x, double& y) {
 
 // rungeKutta3 8:844-862, forward in time; the last stage is not divided by
 // the cell size (8:857-858)
-__global__ void particle_advect_kernel(AdvectPArgs a) {
+template <typename P>
+__global__ void particle_advect_kernel(AdvectPArgs<P> a) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= a.n) return;
     const QtyView &u = a.u, &v = a.v;
@@ -961,8 +979,8 @@ __global__ void particle_advect_kernel(AdvectPArgs a) {
     x += a.dt * ((2.0 / 9.0) * first_u + (3.0 / 9.0) * mid_u + (4.0 / 9.0) * last_u);
     y += a.dt * ((2.0 / 9.0) * first_v + (3.0 / 9.0) * mid_v + (4.0 / 9.0) * last_v);
     particle_back_project(a, x, y);
-    a.posx[i] = smax(smin(x, a.W - 0.001), 0.0);
-    a.posy[i] = smax(smin(y, a.H - 0.001), 0.0);
+    a.posx[i] = (P)smax(smin(x, a.W - 0.001), 0.0);
+    a.posy[i] = (P)smax(smin(y, a.H - 0.001), 0.0);
 }
 
 int bits_for(long n) {
@@ -971,11 +989,21 @@ int bits_for(long n) {
     return b;
 }
 
-ParticleArrays particle_arrays(Solver* s) {
-    ParticleArrays pa;
+// the particle state: double in the drop-in, float in the reduced-precision forms
+bool fp32_particles(const Solver* s) { return s->cfg.precision != IFL_PREC_FP64; }
+template <typename P> ParticleArrays<P> particle_arrays(Solver* s);
+template <> ParticleArrays<double> particle_arrays<double>(Solver* s) {
+    ParticleArrays<double> pa;
     pa.a[0] = s->posx;
     pa.a[1] = s->posy;
     for (int t = 0; t < 4; t++) pa.a[2 + t] = s->prop[t];
+    return pa;
+}
+template <> ParticleArrays<float> particle_arrays<float>(Solver* s) {
+    ParticleArrays<float> pa;
+    pa.a[0] = s->fposx;
+    pa.a[1] = s->fposy;
+    for (int t = 0; t < 4; t++) pa.a[2 + t] = s->fprop[t];
     return pa;
 }
 
@@ -1072,8 +1100,13 @@ int flip_init_particles(Solver* s) {
                                                                    bits);
     IFL_LAUNCH_CHECK();
     IFL_TRY(cub_exclusive_sum(s, cnt, off, n));
-    init_write_kernel<<<grid_blocks(n, 256), 256, 0, s->stream>>>(s->W, s->H, s0, cnt, off, bits, s->posx, s->posy,
-                                                                   dev_sc(s));
+    if (fp32_particles(s)) {
+        init_write_kernel<<<grid_blocks(n, 256), 256, 0, s->stream>>>(s->W, s->H, s0, cnt, off, bits, s->fposx,
+                                                                       s->fposy, dev_sc(s));
+    } else {
+        init_write_kernel<<<grid_blocks(n, 256), 256, 0, s->stream>>>(s->W, s->H, s0, cnt, off, bits, s->posx,
+                                                                       s->posy, dev_sc(s));
+    }
     IFL_LAUNCH_CHECK();
     IFL_TRY(read_scalars(s));
     s->particle_count = host_sc(s)->count;
@@ -1083,16 +1116,18 @@ int flip_init_particles(Solver* s) {
     return IFL_OK;
 }
 
-int k_grid_to_particles(Solver* s, double alpha) {
+template <typename P>
+static int grid_to_particles(Solver* s, double alpha) {
     int n = s->particle_count;
     if (n == 0) return IFL_OK;
-    G2PArgs a;
+    const ParticleArrays<P> pa = particle_arrays<P>(s);
+    G2PArgs<P> a;
     for (int t = 0; t < 4; t++) {
         a.q[t] = s->q[t];
-        a.prop[t] = s->prop[t];
+        a.prop[t] = pa.a[2 + t];
     }
-    a.posx = s->posx;
-    a.posy = s->posy;
+    a.posx = pa.a[0];
+    a.posy = pa.a[1];
     a.n = n;
     a.alpha = alpha;
     IFL_TRY(prof_begin(s, IFL_KERNEL_G2P, -1));
@@ -1100,6 +1135,10 @@ int k_grid_to_particles(Solver* s, double alpha) {
     IFL_LAUNCH_CHECK();
     IFL_TRY(prof_end(s));
     return IFL_OK;
+}
+
+int k_grid_to_particles(Solver* s, double alpha) {
+    return fp32_particles(s) ? grid_to_particles<float>(s, alpha) : grid_to_particles<double>(s, alpha);
 }
 
 int k_copy_old(Solver* s, int qi) {
@@ -1123,32 +1162,42 @@ int k_count_particles(Solver* s) {
     IFL_HIP(hipMemsetAsync(s->counts, 0, cells * sizeof(int), s->stream));
     int n = s->particle_count;
     if (n == 0) return IFL_OK;
-    count_kernel<<<grid_blocks(n, 256), 256, 0, s->stream>>>(s->posx, s->posy, n, s->W, s->H, s->counts);
+    if (fp32_particles(s))
+        count_kernel<<<grid_blocks(n, 256), 256, 0, s->stream>>>(s->fposx, s->fposy, n, s->W, s->H, s->counts);
+    else
+        count_kernel<<<grid_blocks(n, 256), 256, 0, s->stream>>>(s->posx, s->posy, n, s->W, s->H, s->counts);
     IFL_LAUNCH_CHECK();
     return IFL_OK;
 }
 
-int k_prune(Solver* s) {
+template <typename P>
+static int prune(Solver* s) {
     int n = s->particle_count;
     if (n == 0) return IFL_OK;
     IFL_TRY(push_scalars(s));
-    prune_flag_kernel<<<grid_blocks(n, 256), 256, 0, s->stream>>>(s->posx, s->posy, n, s->W, s->H, s->counts, s->pflag);
+    const ParticleArrays<P> pa = particle_arrays<P>(s);
+    prune_flag_kernel<<<grid_blocks(n, 256), 256, 0, s->stream>>>(pa.a[0], pa.a[1], n, s->W, s->H, s->counts,
+                                                                   s->pflag);
     IFL_LAUNCH_CHECK();
     int* cand = s->pidx;
     int* ncand = s->cbuf0;
     IFL_TRY(cub_select(s, s->pflag, cand, ncand, n));
-    prune_walk_kernel<<<1, 64, 0, s->stream>>>(s->posx, s->posy, n, s->W, s->H, s->counts, cand, ncand, s->pidx2,
+    prune_walk_kernel<<<1, 64, 0, s->stream>>>(pa.a[0], pa.a[1], n, s->W, s->H, s->counts, cand, ncand, s->pidx2,
                                                (int*)s->pkey, dev_sc(s));
     IFL_LAUNCH_CHECK();
-    prune_apply_kernel<<<256, 256, 0, s->stream>>>(particle_arrays(s), s->pidx2, (const int*)s->pkey, dev_sc(s));
+    prune_apply_kernel<<<256, 256, 0, s->stream>>>(pa, s->pidx2, (const int*)s->pkey, dev_sc(s));
     IFL_LAUNCH_CHECK();
     IFL_TRY(read_scalars(s));
     s->particle_count = host_sc(s)->count;
     return IFL_OK;
 }
 
-int k_seed(Solver* s) {
+int k_prune(Solver* s) { return fp32_particles(s) ? prune<float>(s) : prune<double>(s); }
+
+template <typename P>
+static int seed(Solver* s) {
     long n = (long)s->W * s->H;
+    const ParticleArrays<P> pa = particle_arrays<P>(s);
     IFL_TRY(push_scalars(s));
     uint32_t s0 = frand_state();
     int* att = s->cbuf0;
@@ -1158,15 +1207,15 @@ int k_seed(Solver* s) {
     seed_attempts_kernel<<<grid_blocks(n, 256), 256, 0, s->stream>>>(s->counts, n, att);
     IFL_LAUNCH_CHECK();
     IFL_TRY(cub_exclusive_sum(s, att, aoff, n));
-    seed_accept_kernel<<<grid_blocks(n, 256), 256, 0, s->stream>>>(s->posx, s->posy, att, n, s->bodies_d, s->nbodies,
+    seed_accept_kernel<<<grid_blocks(n, 256), 256, 0, s->stream>>>(pa.a[0], pa.a[1], att, n, s->bodies_d, s->nbodies,
                                                                     s->hx, dev_sc(s), acc);
     IFL_LAUNCH_CHECK();
     IFL_TRY(cub_exclusive_sum(s, acc, joff, n));
     seed_plan_kernel<<<grid_blocks(n, 256), 256, 0, s->stream>>>(att, aoff, acc, joff, n, s->max_particles, dev_sc(s));
     IFL_LAUNCH_CHECK();
-    SeedArgs a;
+    SeedArgs<P> a;
     for (int t = 0; t < 4; t++) a.q[t] = s->q[t];
-    a.pa = particle_arrays(s);
+    a.pa = pa;
     a.att = att;
     a.aoff = aoff;
     a.acc = acc;
@@ -1188,10 +1237,10 @@ int k_seed(Solver* s) {
     return IFL_OK;
 }
 
-// fromParticles for quantity qi; with qi == IFL_Q_D the temperature, which
-// lives on the same grid, is transferred in the same pass (the weights of a
-// node depend only on the positions).
-// fromParticles of the grids qis[0..nq) (with t on d's grid when with_t)
+int k_seed(Solver* s) { return fp32_particles(s) ? seed<float>(s) : seed<double>(s); }
+
+// fromParticles of the grids qis[0..nq) (with t on d's grid when with_t),
+// from the float particle state
 static int k_from_particles32(Solver* s, const int* qis, int nq, bool with_t) {
     P2G32Args a;
     a.n = s->particle_count;
@@ -1201,7 +1250,7 @@ static int k_from_particles32(Solver* s, const int* qis, int nq, bool with_t) {
     for (int g = 0; g < 3; g++) {
         const int qi = g < nq ? qis[g] : qis[0];
         a.q[g] = s->q[qi];
-        a.prop[g] = s->prop[qi];
+        a.prop[g] = s->fprop[qi];
         a.ntx[g] = (a.q[g].w + kP2GTile - 1) / kP2GTile;
         const int nty = (a.q[g].h + kP2GTile - 1) / kP2GTile;
         a.tile_base[g + 1] = a.tile_base[g] + (g < nq ? a.ntx[g] * nty : 0);
@@ -1209,11 +1258,11 @@ static int k_from_particles32(Solver* s, const int* qis, int nq, bool with_t) {
     }
     const int ntiles = a.tile_base[3];
     const bool t0 = with_t && qis[0] == IFL_Q_D;
-    a.prop2 = t0 ? s->prop[IFL_Q_T] : nullptr;
+    a.prop2 = t0 ? s->fprop[IFL_Q_T] : nullptr;
     a.cell2 = t0 ? s->q[IFL_Q_T].cell : nullptr;
     a.src2 = t0 ? s->q[IFL_Q_T].src : nullptr;
-    a.posx = s->posx;
-    a.posy = s->posy;
+    a.posx = s->fposx;
+    a.posy = s->fposy;
     a.tile_cnt = s->cbuf0;
     a.tile_start = s->cbuf1;
     a.tile_fill = s->cbuf2;
@@ -1244,8 +1293,11 @@ static int k_from_particles32(Solver* s, const int* qis, int nq, bool with_t) {
     return IFL_OK;
 }
 
+// fromParticles for quantity qi; with qi == IFL_Q_D the temperature, which
+// lives on the same grid, is transferred in the same pass (the weights of a
+// node depend only on the positions).
 int k_from_particles(Solver* s, int qi, bool with_t) {
-    if (s->cfg.precision != IFL_PREC_FP64) return k_from_particles32(s, &qi, 1, with_t);
+    if (fp32_particles(s)) return k_from_particles32(s, &qi, 1, with_t);
     const QtyView& q = s->q[qi];
     long cells = (long)q.w * q.h;
     int n = s->particle_count;
@@ -1328,14 +1380,16 @@ int k_particles_to_grid(Solver* s) {
     return IFL_OK;
 }
 
-int k_particle_advect(Solver* s, double dt) {
+template <typename P>
+static int particle_advect(Solver* s, double dt) {
     int n = s->particle_count;
     if (n == 0) return IFL_OK;
-    AdvectPArgs a;
+    const ParticleArrays<P> pa = particle_arrays<P>(s);
+    AdvectPArgs<P> a;
     a.u = s->q[IFL_Q_U];
     a.v = s->q[IFL_Q_V];
-    a.posx = s->posx;
-    a.posy = s->posy;
+    a.posx = pa.a[0];
+    a.posy = pa.a[1];
     a.n = n;
     a.W = s->W;
     a.H = s->H;
@@ -1348,6 +1402,10 @@ int k_particle_advect(Solver* s, double dt) {
     IFL_LAUNCH_CHECK();
     IFL_TRY(prof_end(s));
     return IFL_OK;
+}
+
+int k_particle_advect(Solver* s, double dt) {
+    return fp32_particles(s) ? particle_advect<float>(s, dt) : particle_advect<double>(s, dt);
 }
 
 int flip_check_overflow(Solver* s) {

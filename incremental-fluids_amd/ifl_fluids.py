@@ -117,6 +117,12 @@ def _load():
 lib = _load()
 
 _DTYPE = {8: np.float64, 4: np.int32, 1: np.uint8}
+# the particle state is float in the reduced-precision forms (4-byte elements)
+_FLOAT_FIELDS = frozenset(range(80, 86))
+
+
+def _dtype(fid, e):
+    return np.float32 if e == 4 and fid in _FLOAT_FIELDS else _DTYPE[e]
 
 
 class SolidBody:
@@ -235,13 +241,13 @@ class FluidSolver:
         n, e = self.field_info(fid)
         if n == 0:
             return None
-        a = np.empty(n, dtype=_DTYPE[e])
+        a = np.empty(n, dtype=_dtype(fid, e))
         self._check(lib.ifl_download(self._h, fid, a.ctypes.data, a.nbytes))
         return a
 
     def set_field(self, fid, values):
         n, e = self.field_info(fid)
-        a = np.ascontiguousarray(np.asarray(values, dtype=_DTYPE[e]).reshape(n))
+        a = np.ascontiguousarray(np.asarray(values, dtype=_dtype(fid, e)).reshape(n))
         self._check(lib.ifl_upload(self._h, fid, a.ctypes.data, a.nbytes))
 
     def stage(self, stage, iarg=0, darg=0.0, stats=None):

@@ -1,7 +1,9 @@
-"""Solver 8 in the reduced-precision form: fromParticles (8:663-688) as an
-LDS-binned scatter accumulated in single precision, against the oracle's
-double-precision transfer on the same particles (teacher-forced from the
-oracle's state after a few steps of the reference scene). The sums are
+"""Solver 8 in the reduced-precision form: the particle state is float and
+fromParticles (8:663-688) is an LDS-binned scatter accumulated in single
+precision, checked against the oracle's double-precision transfer on the
+same particles (teacher-forced from the oracle's state after a few steps of
+the reference scene, the oracle's particles rounded to float first so that
+both transfer the same values). The sums are
 taken in no fixed order, so the comparison is to a tolerance: per node
 weight within 2e-6 of the oracle's (relative to max(1, weight)), every
 transferred field within 4e-6 of its largest magnitude, and the EMPTY
@@ -12,7 +14,7 @@ import pytest
 
 import oracle_lib as ol
 from gpu_util import make_product
-from test_gpu_flip import MULTI, push_state, warm
+from test_gpu_flip import MULTI, PARTICLES, push_state, warm
 
 pytestmark = pytest.mark.gpu
 
@@ -29,7 +31,13 @@ def test_from_particles_single_precision(w, h, bodies, warm_steps):
     import ifl_fluids as F
     orc, sc = warm(w, h, bodies, warm_steps)
     mix = make_product(8, w, h, sc["params"], sc["bodies"], precision=F.PREC_MIXED)
+    for f in PARTICLES:
+        assert mix.field_info(f)[1] == 4, "the reduced-precision form keeps the particles in float"
+        orc.set_field(f, orc.field(f).astype(np.float32))
     push_state(orc, mix)
+    for f in PARTICLES:   # the float state holds the rounded values exactly
+        n = orc.particle_count()
+        assert (mix.field(f)[:n] == orc.field(f)[:n]).all()
     for q in range(4):
         orc.stage("FILL_SOLID", q)
         mix.stage("FILL_SOLID", q)
