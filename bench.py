@@ -37,6 +37,16 @@ KERNEL_BYTES_PER_CELL = {
 # the reference's _z at non-FLUID cells (A s, kept in q) for the unmasked dot
 BATCH_BYTES_PER_CELL = dict(KERNEL_BYTES_PER_CELL, sweep_bwd=56.0)
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "traffic.json")
+# The order-preserving floor of a MIC(0) sweep: cell (x, y) needs (x-1, y) and
+# (x, y-1), so any schedule that keeps each cell's operations takes W + H - 1
+# dependent steps, each at least the recurrence's dependency chain (44 clocks,
+# measured in a one-wave loop of the compute step) at the 2.4 GHz engine clock.
+SWEEP_CHAIN_CLOCKS = 44
+ENGINE_GHZ = 2.4
+
+
+def sweep_floor_us(w, h):
+    return (w + h - 1) * SWEEP_CHAIN_CLOCKS / (ENGINE_GHZ * 1e3)
 
 
 def parse():
@@ -619,6 +629,14 @@ def main():
                            "traffic": measured_traffic(dom) if main_form == "fp64" else None,
                            "kernel": f"{dom}: {KERNEL_BYTES_PER_CELL[dom]:.0f} B/cell x {n}x{n} cells per launch, "
                                      f"average over {k['launches']} launches (HIP events on the solver's stream)"}
+        if dom.startswith("sweep"):
+            fl = sweep_floor_us(n, n)
+            # the same kernel against its dependency floor: the fraction of the
+            # fastest order-preserving sweep's rate (HBM peak not reachable)
+            out["roofline"]["latency_floor"] = {
+                "us": fl, "avg_us": k["avg_us"], "frac": fl / k["avg_us"],
+                "achieved_at_floor_GBps": k["bytes_per_launch"] / fl / 1e3,
+                "basis": f"{2 * n - 1} dependent steps x {SWEEP_CHAIN_CLOCKS} clocks at {ENGINE_GHZ} GHz"}
     if len(forms) > 1:
         out["forms"] = {f: {"steps_per_s": res[f]["steps_per_s"], "ms_per_step": res[f]["ms_per_step"],
                             "steps": len(res[f]["solves"]),
