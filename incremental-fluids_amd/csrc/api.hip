@@ -24,6 +24,7 @@ struct ifl_batch {
     int device = 0;
     ifl::BatchDev* dev = nullptr;
     ifl::ExtrapDev* xdev = nullptr;
+    ifl::StageDev* sdev = nullptr;
     bool profiling = false;
     ifl::BatchProfile prof;
 };
@@ -512,6 +513,80 @@ int update_part(Solver* s, int after, double dt, int* next) {
     return IFL_OK;
 }
 
+// update_part for a whole batch of solvers of one level (3-7, bodies
+// evaluated on the device): the same stages in the same order, each one
+// launch for every solver on the batch's stream (kb_*); the extrapolations
+// are STAGE_EXTRAPOLATE (k_extrapolate_batch), except t's after the heat
+// solve, which runs here.
+int update_part_batch(ifl_batch* b, int after, double dt, int* next) {
+    StageDev* d = b->sdev;
+    const int lv = b->solvers[0]->level;
+    *next = SOLVE_NONE;
+    if (after == SOLVE_NONE) {
+        for (Solver* s : b->solvers) {
+            s->border_solid = 0;
+            s->border_pending = false;
+        }
+        if (lv == 3) {
+            IFL_TRY(kb_build_rhs(d));
+            IFL_TRY(kb_build_matrix(d, 0, dt));
+            *next = SOLVE_PRESSURE;
+            return IFL_OK;
+        }
+        IFL_TRY(kb_fill_solid(d, IFL_Q_D));
+        if (lv >= 6) IFL_TRY(kb_fill_solid(d, IFL_Q_T));
+        IFL_TRY(kb_fill_solid(d, IFL_Q_U));
+        IFL_TRY(kb_fill_solid(d, IFL_Q_V));
+        if (lv >= 6) {
+            IFL_TRY(kb_copy_t_to_r(d));
+            IFL_TRY(kb_build_matrix(d, 1, dt));
+            *next = SOLVE_HEAT;
+            return IFL_OK;
+        }
+    }
+    if (after == SOLVE_HEAT) {
+        IFL_TRY(kb_copy_p_to_t(d));
+        IFL_TRY(k_extrapolate_batch(b->solvers.data(), (int)b->solvers.size(), b->stream, &b->xdev, 1 << IFL_Q_T,
+                                    false));
+        IFL_TRY(kb_add_buoyancy(d, dt));
+    }
+    if (after != SOLVE_PRESSURE && after != STAGE_EXTRAPOLATE) {
+        IFL_TRY(kb_set_boundary(d));
+        IFL_TRY(kb_build_rhs(d));
+        if (lv >= 7) IFL_TRY(kb_compute_densities(d));
+        IFL_TRY(kb_build_matrix(d, 0, dt));
+        *next = SOLVE_PRESSURE;
+        return IFL_OK;
+    }
+    if (after == SOLVE_PRESSURE) {
+        IFL_TRY(kb_apply_pressure(d, dt));
+        if (lv == 3) {
+            IFL_TRY(kb_advect(d, (1 << IFL_Q_D) | (1 << IFL_Q_U) | (1 << IFL_Q_V), dt));
+            for (Solver* s : b->solvers) {
+                flip_q(s, IFL_Q_D);
+                flip_q(s, IFL_Q_U);
+                flip_q(s, IFL_Q_V);
+            }
+            return IFL_OK;
+        }
+        *next = STAGE_EXTRAPOLATE;
+        return IFL_OK;
+    }
+    // after the extrapolations of d, u and v: their border flag is read at
+    // the end of the step (kb_finish)
+    for (Solver* s : b->solvers) s->border_pending = true;
+    IFL_TRY(kb_set_boundary(d));
+    const int qm = (1 << IFL_Q_D) | (1 << IFL_Q_U) | (1 << IFL_Q_V) | (lv >= 6 ? (1 << IFL_Q_T) : 0);
+    IFL_TRY(kb_advect(d, qm, dt));
+    for (Solver* s : b->solvers) {
+        flip_q(s, IFL_Q_D);
+        if (lv >= 6) flip_q(s, IFL_Q_T);
+        flip_q(s, IFL_Q_U);
+        flip_q(s, IFL_Q_V);
+    }
+    return IFL_OK;
+}
+
 // the end of a step: the non-finite check, timing collection, statistics
 int finish_step(Solver* s, const ifl_solve_stats& heat, const ifl_solve_stats& press, ifl_step_stats* stats) {
     IFL_TRY(k_nonfinite(s, s->step_flags));
@@ -982,24 +1057,64 @@ int ifl_batch_update(ifl_batch* b, double dt, ifl_step_stats* stats) {
         explicit Batched(std::vector<Solver*>& x) : v(x) { for (Solver* s : v) s->batched = true; }
         ~Batched() { for (Solver* s : v) s->batched = false; }
     } batched(b->solvers);
+    // Levels 3-7 with bodies evaluated on the device run every stage as one
+    // launch for the whole batch on its stream; FLIP (particles) and
+    // host-evaluated bodies run their stages per solver on the solvers' own
+    // streams, joined to the batched solves by events.
+    bool staged = b->solvers[0]->level <= 7;
+    for (Solver* s : b->solvers) staged = staged && !s->host_geom;
+    if (staged) {
+        rc = kb_begin(b->solvers.data(), n, b->stream, &b->sdev);
+        if (rc != IFL_OK) g_err = b->solvers[0]->err;
+    }
     while (rc == IFL_OK) {
         int next = SOLVE_NONE;
-        for (int i = 0; i < n && rc == IFL_OK; i++) {
-            int ni = SOLVE_NONE;
-            rc = update_part(b->solvers[i], after, dt, &ni);
-            if (rc != IFL_OK) g_err = b->solvers[i]->err;
-            next = ni;  // every solver of one level takes the same sequence of solves
+        if (staged) {
+            rc = update_part_batch(b, after, dt, &next);
+            if (rc != IFL_OK) g_err = b->solvers[0]->err;
+        } else {
+            for (int i = 0; i < n && rc == IFL_OK; i++) {
+                int ni = SOLVE_NONE;
+                rc = update_part(b->solvers[i], after, dt, &ni);
+                if (rc != IFL_OK) g_err = b->solvers[i]->err;
+                next = ni;  // every solver of one level takes the same sequence of solves
+            }
         }
         if (rc != IFL_OK || next == SOLVE_NONE) break;
         if (next == STAGE_EXTRAPOLATE)
-            rc = k_extrapolate_batch(b->solvers.data(), n, b->stream, &b->xdev);
+            rc = k_extrapolate_batch(b->solvers.data(), n, b->stream, &b->xdev,
+                                     (1 << IFL_Q_D) | (1 << IFL_Q_U) | (1 << IFL_Q_V), !staged);
         else
             rc = k_project_batch(b->solvers.data(), n, b->stream, &b->dev,
-                                 next == SOLVE_HEAT ? heat.data() : press.data(), b->profiling ? &b->prof : nullptr);
+                                 next == SOLVE_HEAT ? heat.data() : press.data(), b->profiling ? &b->prof : nullptr,
+                                 !staged);
         if (rc != IFL_OK) g_err = b->solvers[0]->err;
         after = next;
     }
-    for (int i = 0; i < n && rc == IFL_OK; i++) {
+    if (staged && rc == IFL_OK) {
+        std::vector<int> nonfinite(n), border(n);
+        rc = kb_finish(b->sdev, b->solvers[0]->level >= 4, nonfinite.data(), border.data());
+        if (rc != IFL_OK) g_err = b->solvers[0]->err;
+        for (int i = 0; i < n && rc == IFL_OK; i++) {
+            Solver* s = b->solvers[i];
+            if (s->border_pending) s->border_solid |= border[i];
+            s->border_pending = false;
+            rc = prof_collect(s, nullptr);
+            if (rc != IFL_OK) {
+                g_err = s->err;
+                break;
+            }
+            if (stats) {
+                stats[i].nonfinite = nonfinite[i];
+                stats[i].reserved = 0;
+                stats[i].heat = heat[i];
+                stats[i].pressure = press[i];
+                stats[i].particle_count = s->particle_count;
+                stats[i].border_solid = s->border_solid;
+            }
+        }
+    }
+    for (int i = 0; i < n && rc == IFL_OK && !staged; i++) {
         rc = finish_step(b->solvers[i], heat[i], press[i], stats ? &stats[i] : nullptr);
         if (rc != IFL_OK) g_err = b->solvers[i]->err;
     }
@@ -1031,6 +1146,7 @@ int ifl_batch_destroy(ifl_batch* b) {
     }
     batch_dev_free(b->dev);
     extrap_dev_free(b->xdev);
+    stage_dev_free(b->sdev);
     delete b;
     return IFL_OK;
 }

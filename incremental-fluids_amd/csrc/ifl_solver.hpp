@@ -214,15 +214,37 @@ int k_matvec(Solver* s);             // z = A s, the stage form
 int k_dot(Solver* s, int which, double* out);  // 0: z.r  1: z.s
 int k_inf_norm(Solver* s, double* out);
 int k_project(Solver* s, int limit, ifl_solve_stats* stats);
-// project() of several solvers of one level at once: each solver's entry and
-// tail on its own stream, the iterations on `stream`, one launch per phase of
-// each PCG iteration for all of them (stats: n entries)
 struct BatchDev;
 struct ExtrapDev;
-// the batched form's extrapolation of d, u and v (extrapolate 7:560-588) for
-// solvers of levels 4-7, one launch per phase for all of them
-int k_extrapolate_batch(Solver* const* ss, int n, hipStream_t stream, ExtrapDev** dev);
+struct StageDev;
+// the batched form's extrapolations (extrapolate 7:560-588) of the quantities
+// in qmask, in d, t, u, v order, for solvers of levels 4-7: one launch per
+// phase for all of them. With `join` the batch's stream first waits for the
+// solvers' own streams and they wait for it afterwards (their stages run on
+// them); without, every stage runs on `stream`.
+int k_extrapolate_batch(Solver* const* ss, int n, hipStream_t stream, ExtrapDev** dev, int qmask, bool join);
 void extrap_dev_free(ExtrapDev* d);
+// The batched form's grid stages of update() (levels 3-7, bodies evaluated on
+// the device): each stage one launch on `stream` for every solver of the
+// batch, the single-solver kernels' bodies over a table of the solvers'
+// arguments. kb_begin (re)binds the solver list, makes `stream` wait for the
+// solvers' own streams and resets the argument tables of the step.
+int kb_begin(Solver* const* ss, int n, hipStream_t stream, StageDev** dev);
+int kb_fill_solid(StageDev* d, int qi);  // fillSolidFields of quantity qi (d: + the FLUID flags)
+int kb_set_boundary(StageDev* d);
+int kb_build_rhs(StageDev* d);
+int kb_compute_densities(StageDev* d);
+int kb_build_matrix(StageDev* d, int heat, double dt);  // buildPressureMatrix / buildHeatDiffusionMatrix
+int kb_apply_pressure(StageDev* d, double dt);
+int kb_add_buoyancy(StageDev* d, double dt);
+int kb_copy_t_to_r(StageDev* d);
+int kb_copy_p_to_t(StageDev* d);
+int kb_advect(StageDev* d, int qmask, double dt);
+// the end of the step: per solver the non-finite flag and (with `border`) the
+// border flag of the last extrapolation, read back with one copy (waits for
+// the stream)
+int kb_finish(StageDev* d, bool border, int* nonfinite, int* border_solid);
+void stage_dev_free(StageDev* d);
 // per-kernel totals of a batch's fused PCG launches (when profiling): every
 // launch timed with events on the batch's stream; cell_iterations counts the
 // cells of the instances that did work in them
@@ -231,8 +253,13 @@ struct BatchProfile {
     double ms[IFL_KERNEL_COUNT] = {};
     double cell_iterations[IFL_KERNEL_COUNT] = {};
 };
+// project() of several solvers of one level at once, one launch per phase
+// of each PCG iteration for all of them (stats: n entries). With `join` the
+// solvers' stages run on their own streams, joined by events, and each
+// solver's exit tail runs on its stream; without, the tails too run on
+// `stream`, as one launch.
 int k_project_batch(Solver* const* ss, int n, hipStream_t stream, BatchDev** dev, ifl_solve_stats* stats,
-                    BatchProfile* prof);
+                    BatchProfile* prof, bool join);
 void batch_dev_free(BatchDev* d);
 // per-launch timing (when profiling): events on the solver's stream around
 // a launch; prof_collect() adds the elapsed times of launches whose index is

@@ -14,10 +14,10 @@ namespace ifl {
 
 // ------------------------------------------------------------ fillSolidFields
 // phi at the (w+1)(h+1) corners, 7:479-488
-__global__ void fill_phi_kernel(QtyView q, const Body* __restrict__ bodies, int nb) {
+IFL_DEV void fill_phi_body(const QtyView& q, const Body* __restrict__ bodies, int nb, long gi) {
     int w1 = q.w + 1, h1 = q.h + 1;
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= w1 * h1) return;
+    if (gi >= (long)w1 * h1) return;
+    int i = (int)gi;
     int ix = i % w1, iy = i / w1;
     double x = (ix + q.ox - 0.5) * q.hx;
     double y = (iy + q.oy - 0.5) * q.hx;
@@ -25,11 +25,14 @@ __global__ void fill_phi_kernel(QtyView q, const Body* __restrict__ bodies, int 
     for (int k = 1; k < nb; k++) phi = smin(phi, body_distance(bodies[k], x, y));
     q.phi[i] = phi;
 }
+__global__ void fill_phi_kernel(QtyView q, const Body* __restrict__ bodies, int nb) {
+    fill_phi_body(q, bodies, nb, blockIdx.x * (long)blockDim.x + threadIdx.x);
+}
 
 // body, volume, normal, cell per cell: 4:453-479 (level 4), 7:490-521 (5+)
-__global__ void fill_cells_kernel(QtyView q, const Body* __restrict__ bodies, int nb, int level) {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= q.w * q.h) return;
+IFL_DEV void fill_cells_body(const QtyView& q, const Body* __restrict__ bodies, int nb, int level, long gi) {
+    if (gi >= (long)q.w * q.h) return;
+    int i = (int)gi;
     int ix = i % q.w, iy = i / q.w;
     double x = (ix + q.ox) * q.hx;
     double y = (iy + q.oy) * q.hx;
@@ -59,6 +62,9 @@ __global__ void fill_cells_kernel(QtyView q, const Body* __restrict__ bodies, in
     q.nx[i] = nx;
     q.ny[i] = ny;
     q.cell[i] = (vol == 0.0) ? IFL_CELL_SOLID : IFL_CELL_FLUID;
+}
+__global__ void fill_cells_kernel(QtyView q, const Body* __restrict__ bodies, int nb, int level) {
+    fill_cells_body(q, bodies, nb, level, blockIdx.x * (long)blockDim.x + threadIdx.x);
 }
 
 // fillSolidFields with host-evaluated bodies (7:475-522): phi, the closest
@@ -312,18 +318,19 @@ __global__ __launch_bounds__(1024) void batch_extrapolate_levels_kernel(const Ex
 
 struct ExtrapDev {
     std::vector<Solver*> solvers;
-    ExtrapInst* inst_h[3] = {nullptr, nullptr, nullptr};  // pinned, one table per quantity
-    ExtrapInst* inst_d[3] = {nullptr, nullptr, nullptr};
-    int2* mask_map[3] = {nullptr, nullptr, nullptr};
-    int2* border_map[3] = {nullptr, nullptr, nullptr};
-    int nmask[3] = {0, 0, 0}, nborder[3] = {0, 0, 0};
+    // one table per quantity (indexed by IFL_Q_*), built on first use
+    ExtrapInst* inst_h[4] = {nullptr, nullptr, nullptr, nullptr};  // pinned
+    ExtrapInst* inst_d[4] = {nullptr, nullptr, nullptr, nullptr};
+    int2* mask_map[4] = {nullptr, nullptr, nullptr, nullptr};
+    int2* border_map[4] = {nullptr, nullptr, nullptr, nullptr};
+    int nmask[4] = {0, 0, 0, 0}, nborder[4] = {0, 0, 0, 0};
     std::vector<hipEvent_t> ready;
     hipEvent_t joined = nullptr;
 };
 
 void extrap_dev_free(ExtrapDev* d) {
     if (!d) return;
-    for (int k = 0; k < 3; k++) {
+    for (int k = 0; k < 4; k++) {
         if (d->inst_h[k]) (void)hipHostFree(d->inst_h[k]);
         if (d->inst_d[k]) (void)hipFree(d->inst_d[k]);
         if (d->mask_map[k]) (void)hipFree(d->mask_map[k]);
@@ -334,50 +341,54 @@ void extrap_dev_free(ExtrapDev* d) {
     delete d;
 }
 
-int k_extrapolate_batch(Solver* const* ss, int n, hipStream_t stream, ExtrapDev** dev) {
-    static const int kQ[3] = {IFL_Q_D, IFL_Q_U, IFL_Q_V};
+int k_extrapolate_batch(Solver* const* ss, int n, hipStream_t stream, ExtrapDev** dev, int qmask, bool join) {
     Solver* s = ss[0];  // error reporting
     ExtrapDev* d = *dev;
     if (!d || d->solvers.size() != (size_t)n || !std::equal(ss, ss + n, d->solvers.begin())) {
         extrap_dev_free(d);
         d = *dev = new ExtrapDev();
         d->solvers.assign(ss, ss + n);
-        for (int k = 0; k < 3; k++) {
-            std::vector<int2> mm, bm;
-            for (int i = 0; i < n; i++) {
-                const QtyView& q = ss[i]->q[kQ[k]];
-                const long cells = (long)q.w * q.h, perim = 2L * q.w + 2L * q.h;
-                for (long b = 0; b < (cells + 255) / 256; b++) mm.push_back(make_int2(i, (int)b));
-                for (long b = 0; b < (perim + 255) / 256; b++) bm.push_back(make_int2(i, (int)b));
-            }
-            d->nmask[k] = (int)mm.size();
-            d->nborder[k] = (int)bm.size();
-            IFL_HIP(hipMalloc((void**)&d->mask_map[k], mm.size() * sizeof(int2)));
-            IFL_HIP(hipMemcpy(d->mask_map[k], mm.data(), mm.size() * sizeof(int2), hipMemcpyHostToDevice));
-            IFL_HIP(hipMalloc((void**)&d->border_map[k], bm.size() * sizeof(int2)));
-            IFL_HIP(hipMemcpy(d->border_map[k], bm.data(), bm.size() * sizeof(int2), hipMemcpyHostToDevice));
-            IFL_HIP(hipHostMalloc((void**)&d->inst_h[k], n * sizeof(ExtrapInst), hipHostMallocDefault));
-            IFL_HIP(hipMalloc((void**)&d->inst_d[k], n * sizeof(ExtrapInst)));
-        }
         d->ready.assign(n, nullptr);
         for (int i = 0; i < n; i++) IFL_HIP(hipEventCreateWithFlags(&d->ready[i], hipEventDisableTiming));
         IFL_HIP(hipEventCreateWithFlags(&d->joined, hipEventDisableTiming));
     }
+    for (int k = 0; k < 4; k++) {
+        if (!(qmask & (1 << k)) || d->inst_d[k]) continue;
+        std::vector<int2> mm, bm;
+        for (int i = 0; i < n; i++) {
+            const QtyView& q = ss[i]->q[k];
+            const long cells = (long)q.w * q.h, perim = 2L * q.w + 2L * q.h;
+            for (long b = 0; b < (cells + 255) / 256; b++) mm.push_back(make_int2(i, (int)b));
+            for (long b = 0; b < (perim + 255) / 256; b++) bm.push_back(make_int2(i, (int)b));
+        }
+        d->nmask[k] = (int)mm.size();
+        d->nborder[k] = (int)bm.size();
+        IFL_HIP(hipMalloc((void**)&d->mask_map[k], mm.size() * sizeof(int2)));
+        IFL_HIP(hipMemcpy(d->mask_map[k], mm.data(), mm.size() * sizeof(int2), hipMemcpyHostToDevice));
+        IFL_HIP(hipMalloc((void**)&d->border_map[k], bm.size() * sizeof(int2)));
+        IFL_HIP(hipMemcpy(d->border_map[k], bm.data(), bm.size() * sizeof(int2), hipMemcpyHostToDevice));
+        IFL_HIP(hipHostMalloc((void**)&d->inst_h[k], n * sizeof(ExtrapInst), hipHostMallocDefault));
+        IFL_HIP(hipMalloc((void**)&d->inst_d[k], n * sizeof(ExtrapInst)));
+    }
     // the solvers' src arrays swap every step, so the tables are written
     // afresh (the previous step's copies completed before it returned)
-    for (int k = 0; k < 3; k++)
+    for (int k = 0; k < 4; k++) {
+        if (!(qmask & (1 << k))) continue;
         for (int i = 0; i < n; i++) {
             ExtrapInst& e = d->inst_h[k][i];
-            e.q = ss[i]->q[kQ[k]];
+            e.q = ss[i]->q[k];
             e.front = ss[i]->front;
             e.front_cnt = ss[i]->front_cnt;
             e.cap = (long)(ss[i]->W + 1) * (ss[i]->H + 1);
         }
-    for (int i = 0; i < n; i++) {
-        IFL_HIP(hipEventRecord(d->ready[i], ss[i]->stream));
-        IFL_HIP(hipStreamWaitEvent(stream, d->ready[i], 0));
     }
-    for (int k = 0; k < 3; k++) {
+    if (join)
+        for (int i = 0; i < n; i++) {
+            IFL_HIP(hipEventRecord(d->ready[i], ss[i]->stream));
+            IFL_HIP(hipStreamWaitEvent(stream, d->ready[i], 0));
+        }
+    for (int k = 0; k < 4; k++) {
+        if (!(qmask & (1 << k))) continue;
         IFL_HIP(hipMemcpyAsync(d->inst_d[k], d->inst_h[k], n * sizeof(ExtrapInst), hipMemcpyHostToDevice, stream));
         batch_extrap_zero_kernel<<<grid_blocks(n, 256), 256, 0, stream>>>(d->inst_d[k], n);
         IFL_LAUNCH_CHECK();
@@ -388,8 +399,10 @@ int k_extrapolate_batch(Solver* const* ss, int n, hipStream_t stream, ExtrapDev*
         batch_extrapolate_levels_kernel<<<n, 1024, 0, stream>>>(d->inst_d[k]);
         IFL_LAUNCH_CHECK();
     }
-    IFL_HIP(hipEventRecord(d->joined, stream));
-    for (int i = 0; i < n; i++) IFL_HIP(hipStreamWaitEvent(ss[i]->stream, d->joined, 0));
+    if (join) {
+        IFL_HIP(hipEventRecord(d->joined, stream));
+        for (int i = 0; i < n; i++) IFL_HIP(hipStreamWaitEvent(ss[i]->stream, d->joined, 0));
+    }
     return IFL_OK;
 }
 
@@ -557,10 +570,10 @@ struct AdvectArgs {
 
 // rungeKutta3 7:297-315 + backProject 7:424-435 + cerp 7:404-422, as called
 // from advect 7:437-454 (level 3: 3:154-165, all cells, no back-projection)
-__global__ void advect_kernel(AdvectArgs a) {
-    const QtyView& q = a.q[blockIdx.y];
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= q.w * q.h) return;
+IFL_DEV void advect_body(const AdvectArgs& a, int qy, long gi) {
+    const QtyView& q = a.q[qy];
+    if (gi >= (long)q.w * q.h) return;
+    int i = (int)gi;
     if (a.level >= 4 && q.cell[i] != IFL_CELL_FLUID) return;
     int ix = i % q.w, iy = i / q.w;
     double x = ix + q.ox;
@@ -587,8 +600,8 @@ __global__ void advect_kernel(AdvectArgs a) {
             y = (y - q.oy) * hx;
             if (a.req) {
                 // the host evaluates closestSurfacePoint; advect_finish_kernel samples
-                const int k = atomicAdd(&a.req_cnt[blockIdx.y], 1);
-                HostReq* r = a.req + blockIdx.y * a.req_cap + k;
+                const int k = atomicAdd(&a.req_cnt[qy], 1);
+                HostReq* r = a.req + qy * a.req_cap + k;
                 r->idx = i;
                 r->body = q.body[ri];
                 r->x = x;
@@ -602,6 +615,7 @@ __global__ void advect_kernel(AdvectArgs a) {
     }
     q.dst[i] = qcerp(q.src, q.w, q.h, q.ox, q.oy, x, y);
 }
+__global__ void advect_kernel(AdvectArgs a) { advect_body(a, blockIdx.y, blockIdx.x * (long)blockDim.x + threadIdx.x); }
 
 // The mixed form's advection: the same trace (rungeKutta3 with its undivided
 // last stage, backProject, cerp) with every velocity sample and the final
@@ -803,10 +817,10 @@ struct RhsArgs {
     const Body* bodies;
 };
 
-__global__ void build_rhs_kernel(RhsArgs a) {
+IFL_DEV void build_rhs_body(const RhsArgs& a, long gi) {
     int W = a.sk.W, H = a.sk.H;
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= W * H) return;
+    if (gi >= (long)W * H) return;
+    int i = (int)gi;
     int x = i % W, y = i / W;
     int uw = W + 1;
     double scale = 1.0 / a.hx;
@@ -831,6 +845,7 @@ __global__ void build_rhs_kernel(RhsArgs a) {
     }
     a.r[a.sk.pos(x, y)] = res;
 }
+__global__ void build_rhs_kernel(RhsArgs a) { build_rhs_body(a, blockIdx.x * (long)blockDim.x + threadIdx.x); }
 
 int k_build_rhs(Solver* s) {
     RhsArgs a;
@@ -860,9 +875,8 @@ IFL_DEV double cell_density(const double* t, const double* d, int i, double rho_
     return smax(density, 0.05 * rho_air);
 }
 
-__global__ void densities_kernel(const double* __restrict__ t, const double* __restrict__ d, double* __restrict__ ud,
-                                 double* __restrict__ vd, int W, int H, double rho_air, double t_amb, double alpha) {
-    long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
+IFL_DEV void densities_body(const double* __restrict__ t, const double* __restrict__ d, double* __restrict__ ud,
+                           double* __restrict__ vd, int W, int H, double rho_air, double t_amb, double alpha, long i) {
     long nu = (long)(W + 1) * H;
     long nv = (long)W * (H + 1);
     if (i < nu) {
@@ -879,6 +893,10 @@ __global__ void densities_kernel(const double* __restrict__ t, const double* __r
         if (y < H) acc += 0.5 * cell_density(t, d, x + y * W, rho_air, t_amb, alpha);
         vd[k] = acc;
     }
+}
+__global__ void densities_kernel(const double* __restrict__ t, const double* __restrict__ d, double* __restrict__ ud,
+                                 double* __restrict__ vd, int W, int H, double rho_air, double t_amb, double alpha) {
+    densities_body(t, d, ud, vd, W, H, rho_air, t_amb, alpha, blockIdx.x * (long)blockDim.x + threadIdx.x);
 }
 
 int k_compute_densities(Solver* s) {
@@ -934,10 +952,10 @@ IFL_DEV double factor_y(const MatArgs& a, int x, int y) {
     return a.scale * a.vvol[x + (y + 1) * W] / a.vd[x + (y + 1) * (W + 1)];  // u-grid stride, Q2 (7:700)
 }
 
-__global__ void matrix_kernel(MatArgs a) {
+IFL_DEV void matrix_body(const MatArgs& a, long gi) {
     int W = a.sk.W, H = a.sk.H;
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= W * H) return;
+    if (gi >= (long)W * H) return;
+    int i = (int)gi;
     int x = i % W, y = i / W;
     bool fluid = a.level == 3 || a.cell[i] == IFL_CELL_FLUID;
     double diag = a.heat ? 1.0 : 0.0;
@@ -961,6 +979,7 @@ __global__ void matrix_kernel(MatArgs a) {
     a.apx[p] = px;
     a.apy[p] = py;
 }
+__global__ void matrix_kernel(MatArgs a) { matrix_body(a, blockIdx.x * (long)blockDim.x + threadIdx.x); }
 
 static int launch_matrix(Solver* s, int heat, double scale) {
     MatArgs a;
@@ -1011,9 +1030,8 @@ IFL_DEV bool pcell_on(const PressArgs& a, int x, int y) {
     return a.level == 3 || a.cell[x + y * a.sk.W] == IFL_CELL_FLUID;
 }
 
-__global__ void apply_pressure_kernel(PressArgs a) {
+IFL_DEV void apply_pressure_body(const PressArgs& a, long i) {
     int W = a.sk.W, H = a.sk.H;
-    long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
     long nu = (long)(W + 1) * H;
     long nv = (long)W * (H + 1);
     const bool dens = a.level >= 7;
@@ -1046,6 +1064,7 @@ __global__ void apply_pressure_kernel(PressArgs a) {
         a.v[k] = v;
     }
 }
+__global__ void apply_pressure_kernel(PressArgs a) { apply_pressure_body(a, blockIdx.x * (long)blockDim.x + threadIdx.x); }
 
 int k_apply_pressure(Solver* s, double dt) {
     PressArgs a;
@@ -1066,9 +1085,8 @@ int k_apply_pressure(Solver* s, double dt) {
 
 // ------------------------------------------------------------ addBuoyancy 7:908-919
 // v(x, y) += b(x, y-1)*0.5 and then b(x, y)*0.5, loop order.
-__global__ void buoyancy_kernel(double* __restrict__ v, const double* __restrict__ d, const double* __restrict__ t, int W,
-                                int H, double dtg, double alpha, double t_amb) {
-    long k = blockIdx.x * (long)blockDim.x + threadIdx.x;
+IFL_DEV void buoyancy_body(double* __restrict__ v, const double* __restrict__ d, const double* __restrict__ t, int W,
+                           int H, double dtg, double alpha, double t_amb, long k) {
     if (k >= (long)W * (H + 1)) return;
     int x = (int)(k % W), y = (int)(k / W);
     double val = v[k];
@@ -1084,6 +1102,10 @@ __global__ void buoyancy_kernel(double* __restrict__ v, const double* __restrict
     }
     v[k] = val;
 }
+__global__ void buoyancy_kernel(double* __restrict__ v, const double* __restrict__ d, const double* __restrict__ t, int W,
+                                int H, double dtg, double alpha, double t_amb) {
+    buoyancy_body(v, d, t, W, H, dtg, alpha, t_amb, blockIdx.x * (long)blockDim.x + threadIdx.x);
+}
 
 int k_add_buoyancy(Solver* s, double dt) {
     double rho = s->cfg.density;
@@ -1098,9 +1120,9 @@ int k_add_buoyancy(Solver* s, double dt) {
 // ------------------------------------------------------------ setBoundaryCondition 7:921-942
 // A face shared by two SOLID cells takes the value written last in loop
 // order (cell (x, y) after (x-1, y) / (x, y-1)); walls are zeroed last.
-__global__ void boundary_kernel(double* __restrict__ u, double* __restrict__ v, const uint8_t* __restrict__ cell,
-                                const uint8_t* __restrict__ body, const Body* __restrict__ bodies, int W, int H, double hx) {
-    long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
+IFL_DEV void boundary_body(double* __restrict__ u, double* __restrict__ v, const uint8_t* __restrict__ cell,
+                           const uint8_t* __restrict__ body, const Body* __restrict__ bodies, int W, int H, double hx,
+                           long i) {
     long nu = (long)(W + 1) * H;
     long nv = (long)W * (H + 1);
     if (i < nu) {
@@ -1126,6 +1148,10 @@ __global__ void boundary_kernel(double* __restrict__ u, double* __restrict__ v, 
         if (c >= 0) v[k] = body_vely(bodies[body[c]], (x + 0.5) * hx);
     }
 }
+__global__ void boundary_kernel(double* __restrict__ u, double* __restrict__ v, const uint8_t* __restrict__ cell,
+                                const uint8_t* __restrict__ body, const Body* __restrict__ bodies, int W, int H, double hx) {
+    boundary_body(u, v, cell, body, bodies, W, H, hx, blockIdx.x * (long)blockDim.x + threadIdx.x);
+}
 
 int k_set_boundary(Solver* s) {
     long n = (long)(s->W + 1) * s->H + (long)s->W * (s->H + 1);
@@ -1137,15 +1163,21 @@ int k_set_boundary(Solver* s) {
 }
 
 // ------------------------------------------------------------ layout conversion
-__global__ void skew_to_rows_kernel(const double* __restrict__ sk_arr, double* __restrict__ rows, Skew sk) {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= sk.W * sk.H) return;
+IFL_DEV void skew_to_rows_body(const double* __restrict__ sk_arr, double* __restrict__ rows, const Skew& sk, long gi) {
+    if (gi >= (long)sk.W * sk.H) return;
+    int i = (int)gi;
     rows[i] = sk_arr[sk.pos(i % sk.W, i / sk.W)];
 }
-__global__ void rows_to_skew_kernel(const double* __restrict__ rows, double* __restrict__ sk_arr, Skew sk) {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= sk.W * sk.H) return;
+IFL_DEV void rows_to_skew_body(const double* __restrict__ rows, double* __restrict__ sk_arr, const Skew& sk, long gi) {
+    if (gi >= (long)sk.W * sk.H) return;
+    int i = (int)gi;
     sk_arr[sk.pos(i % sk.W, i / sk.W)] = rows[i];
+}
+__global__ void skew_to_rows_kernel(const double* __restrict__ sk_arr, double* __restrict__ rows, Skew sk) {
+    skew_to_rows_body(sk_arr, rows, sk, blockIdx.x * (long)blockDim.x + threadIdx.x);
+}
+__global__ void rows_to_skew_kernel(const double* __restrict__ rows, double* __restrict__ sk_arr, Skew sk) {
+    rows_to_skew_body(rows, sk_arr, sk, blockIdx.x * (long)blockDim.x + threadIdx.x);
 }
 
 int k_skew_to_rows(Solver* s, const double* skew, double* rows) {
@@ -1163,19 +1195,25 @@ int k_rows_to_skew(Solver* s, const double* rows, double* skew) {
 int k_copy_t_to_r(Solver* s) { return k_rows_to_skew(s, s->q[IFL_Q_T].src, s->r); }
 int k_copy_p_to_t(Solver* s) { return k_skew_to_rows(s, s->p, s->q[IFL_Q_T].src); }
 
-__global__ void dflag_kernel(const uint8_t* __restrict__ cell, uint8_t* __restrict__ dflag, Skew sk, int all_fluid) {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= sk.W * sk.H) return;
+IFL_DEV void dflag_body(const uint8_t* __restrict__ cell, uint8_t* __restrict__ dflag, const Skew& sk, int all_fluid,
+                        long gi) {
+    if (gi >= (long)sk.W * sk.H) return;
+    int i = (int)gi;
     dflag[sk.pos(i % sk.W, i / sk.W)] = (all_fluid || cell[i] == IFL_CELL_FLUID) ? 1 : 0;
+}
+__global__ void dflag_kernel(const uint8_t* __restrict__ cell, uint8_t* __restrict__ dflag, Skew sk, int all_fluid) {
+    dflag_body(cell, dflag, sk, all_fluid, blockIdx.x * (long)blockDim.x + threadIdx.x);
 }
 
 // One wave per strip column: the column's 64 flags as a lane mask, the
 // form in which the sweeps select per lane without vector work.
-__global__ __launch_bounds__(256) void fmask_kernel(const uint8_t* __restrict__ dflag, uint64_t* __restrict__ fmask, long n) {
-    long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
+IFL_DEV void fmask_body(const uint8_t* __restrict__ dflag, uint64_t* __restrict__ fmask, long n, long i) {
     bool f = i < n && dflag[i] != 0;
     uint64_t m = __ballot(f);
     if ((threadIdx.x & 63) == 0 && i < n) fmask[i >> 6] = m;
+}
+__global__ __launch_bounds__(256) void fmask_kernel(const uint8_t* __restrict__ dflag, uint64_t* __restrict__ fmask, long n) {
+    fmask_body(dflag, fmask, n, blockIdx.x * (long)blockDim.x + threadIdx.x);
 }
 
 int k_update_dflag(Solver* s) {
@@ -1185,6 +1223,503 @@ int k_update_dflag(Solver* s) {
     long n = (long)s->sk.nstrips * s->sk.strip_sz;
     fmask_kernel<<<grid_blocks(n, 256), 256, 0, s->stream>>>(s->dflag, s->fmask, n);
     IFL_LAUNCH_CHECK();
+    return IFL_OK;
+}
+
+
+// ------------------------------------------------------------ batched stages
+// The batched form's grid stages: each stage of update() is one launch over
+// a table of every solver's arguments, each workgroup mapped to (solver,
+// block of that solver's grid; a second index for the quantity where the
+// single-solver kernel has one) by a table built once per grid shape. The
+// bodies are the single-solver kernels', on the same arrays, so every value
+// is the one the solver would compute stepping alone.
+struct FillArgs {
+    QtyView q;
+    const Body* bodies;
+    int nb, level;
+};
+struct DflagArgs {
+    const uint8_t* cell;
+    uint8_t* dflag;
+    uint64_t* fmask;
+    Skew sk;
+    long n;
+    int all_fluid;
+};
+struct BoundArgs {
+    double *u, *v;
+    const uint8_t *cell, *body;
+    const Body* bodies;
+    int W, H;
+    double hx;
+};
+struct DensArgs {
+    const double *t, *d;
+    double *ud, *vd;
+    int W, H;
+    double rho_air, t_amb, alpha;
+};
+struct BuoyArgs {
+    double* v;
+    const double *d, *t;
+    int W, H;
+    double dtg, alpha, t_amb;
+};
+struct CopyArgs {
+    const double* src;
+    double* dst;
+    Skew sk;
+};
+struct FinishArgs {
+    const double* src[4];  // the quantities checked (by the second index)
+    long n[4];
+    int* flags;            // this solver's two flags: non-finite, border
+    const int* border;     // the border flag of its last extrapolation, or null
+};
+
+#define IFL_BATCH_STAGE(name, A, call)                                                                   \
+    __global__ __launch_bounds__(256) void name(const A* __restrict__ t, const int2* __restrict__ map) { \
+        const int2 m = map[blockIdx.x];                                                                   \
+        const A& a = t[m.x];                                                                              \
+        const long gi = (long)(m.y & 0xffffff) * 256 + threadIdx.x;                                       \
+        const int by = m.y >> 24;                                                                         \
+        (void)by;                                                                                         \
+        call;                                                                                             \
+    }
+
+IFL_DEV void nonfinite_body(const FinishArgs& a, int by, long gi) {
+    const bool bad = gi < a.n[by] && !isfinite(a.src[by][gi]);
+    if (__any(bad) && (threadIdx.x & 63) == 0) atomicOr(a.flags, 1);
+}
+
+IFL_BATCH_STAGE(bs_fill_phi_kernel, FillArgs, fill_phi_body(a.q, a.bodies, a.nb, gi))
+IFL_BATCH_STAGE(bs_fill_cells_kernel, FillArgs, fill_cells_body(a.q, a.bodies, a.nb, a.level, gi))
+IFL_BATCH_STAGE(bs_dflag_kernel, DflagArgs, dflag_body(a.cell, a.dflag, a.sk, a.all_fluid, gi))
+IFL_BATCH_STAGE(bs_fmask_kernel, DflagArgs, fmask_body(a.dflag, a.fmask, a.n, gi))
+IFL_BATCH_STAGE(bs_boundary_kernel, BoundArgs, boundary_body(a.u, a.v, a.cell, a.body, a.bodies, a.W, a.H, a.hx, gi))
+IFL_BATCH_STAGE(bs_rhs_kernel, RhsArgs, build_rhs_body(a, gi))
+IFL_BATCH_STAGE(bs_densities_kernel, DensArgs,
+                densities_body(a.t, a.d, a.ud, a.vd, a.W, a.H, a.rho_air, a.t_amb, a.alpha, gi))
+IFL_BATCH_STAGE(bs_matrix_kernel, MatArgs, matrix_body(a, gi))
+IFL_BATCH_STAGE(bs_pressure_kernel, PressArgs, apply_pressure_body(a, gi))
+IFL_BATCH_STAGE(bs_buoyancy_kernel, BuoyArgs, buoyancy_body(a.v, a.d, a.t, a.W, a.H, a.dtg, a.alpha, a.t_amb, gi))
+IFL_BATCH_STAGE(bs_rows_to_skew_kernel, CopyArgs, rows_to_skew_body(a.src, a.dst, a.sk, gi))
+IFL_BATCH_STAGE(bs_skew_to_rows_kernel, CopyArgs, skew_to_rows_body(a.src, a.dst, a.sk, gi))
+IFL_BATCH_STAGE(bs_advect_kernel, AdvectArgs, advect_body(a, by, gi))
+IFL_BATCH_STAGE(bs_nonfinite_kernel, FinishArgs, nonfinite_body(a, by, gi))
+
+__global__ void bs_flags_kernel(const FinishArgs* __restrict__ t, int n) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    t[i].flags[0] = 0;
+    t[i].flags[1] = t[i].border ? *t[i].border : 0;
+}
+
+// A block table: for each second index y, each solver's blocks of 256
+// elements, counts[y * n + i] elements of solver i. Kept while the counts
+// stay the same.
+struct StageMap {
+    std::vector<long> counts;
+    int2* map = nullptr;
+    int nblk = 0;
+};
+
+struct StageDev {
+    std::vector<Solver*> solvers;
+    hipStream_t stream = nullptr;
+    char* arena_h = nullptr;  // pinned: the step's argument tables
+    char* arena_d = nullptr;
+    size_t cap = 0, off = 0;
+    std::vector<StageMap> maps;
+    int* flags_d = nullptr;   // 2 per solver
+    int* flags_h = nullptr;
+    std::vector<hipEvent_t> ready;
+};
+
+void stage_dev_free(StageDev* d) {
+    if (!d) return;
+    if (d->arena_h) (void)hipHostFree(d->arena_h);
+    if (d->arena_d) (void)hipFree(d->arena_d);
+    for (StageMap& m : d->maps)
+        if (m.map) (void)hipFree(m.map);
+    if (d->flags_d) (void)hipFree(d->flags_d);
+    if (d->flags_h) (void)hipHostFree(d->flags_h);
+    for (hipEvent_t e : d->ready) (void)hipEventDestroy(e);
+    delete d;
+}
+
+namespace {
+
+// the block table of map slot `slot` for these counts (ny second indices)
+int stage_map(StageDev* d, int slot, const std::vector<long>& counts, const StageMap** out) {
+    Solver* s = d->solvers[0];  // error reporting
+    if ((int)d->maps.size() <= slot) d->maps.resize(slot + 1);
+    StageMap& m = d->maps[slot];
+    if (m.map && m.counts == counts) {
+        *out = &m;
+        return IFL_OK;
+    }
+    const int n = (int)d->solvers.size(), ny = (int)(counts.size() / n);
+    std::vector<int2> h;
+    for (int y = 0; y < ny; y++)
+        for (int i = 0; i < n; i++)
+            for (long b = 0; b < (counts[(size_t)y * n + i] + 255) / 256; b++) h.push_back(make_int2(i, (int)b | (y << 24)));
+    if (m.map) IFL_HIP(hipFree(m.map));
+    m.map = nullptr;
+    m.nblk = 0;
+    m.counts.clear();
+    if (!h.empty()) {
+        IFL_HIP(hipMalloc((void**)&m.map, h.size() * sizeof(int2)));
+        IFL_HIP(hipMemcpy(m.map, h.data(), h.size() * sizeof(int2), hipMemcpyHostToDevice));
+    }
+    m.nblk = (int)h.size();
+    m.counts = counts;
+    *out = &m;
+    return IFL_OK;
+}
+
+// the step's table of A for every solver, written on the host and copied to
+// the device on the stream (the arena is reused from the next step on, after
+// the step has waited for the stream)
+template <typename A, typename F>
+int stage_table(StageDev* d, F fill, const A** out) {
+    Solver* s = d->solvers[0];
+    const int n = (int)d->solvers.size();
+    const size_t at = (d->off + 255) & ~(size_t)255, bytes = n * sizeof(A);
+    if (at + bytes > d->cap) {
+        s->err = "batched stages: argument tables exceed their arena";
+        return IFL_ERR_ALLOC;
+    }
+    A* h = (A*)(d->arena_h + at);
+    for (int i = 0; i < n; i++) fill(d->solvers[i], h[i]);
+    IFL_HIP(hipMemcpyAsync(d->arena_d + at, h, bytes, hipMemcpyHostToDevice, d->stream));
+    d->off = at + bytes;
+    *out = (const A*)(d->arena_d + at);
+    return IFL_OK;
+}
+
+// map slots
+enum {
+    MS_CELLS = 0,   // W*H
+    MS_FACES,       // (W+1)*H + W*(H+1)
+    MS_VFACES,      // W*(H+1)
+    MS_SKEW,        // the skewed arrays' length
+    MS_FILL,        // + qi: q.w*q.h of solvers with bodies
+    MS_PHI = MS_FILL + 4,  // + qi: (q.w+1)*(q.h+1) of solvers with bodies (levels 5+)
+    MS_ADVECT = MS_PHI + 4,  // + qmask
+    MS_NONFINITE = MS_ADVECT + 16,
+    MS_COUNT
+};
+
+template <typename G>
+std::vector<long> per_solver(StageDev* d, G g) {
+    std::vector<long> c;
+    for (Solver* s : d->solvers) c.push_back(g(s));
+    return c;
+}
+
+#define IFL_BS_LAUNCH(kernel, tab, m)                                           \
+    do {                                                                         \
+        if ((m)->nblk > 0) {                                                     \
+            kernel<<<(m)->nblk, 256, 0, d->stream>>>(tab, (m)->map);             \
+            IFL_LAUNCH_CHECK();                                                  \
+        }                                                                        \
+    } while (0)
+
+}  // namespace
+
+int kb_begin(Solver* const* ss, int n, hipStream_t stream, StageDev** dev) {
+    Solver* s = ss[0];
+    StageDev* d = *dev;
+    if (!d || d->solvers.size() != (size_t)n || !std::equal(ss, ss + n, d->solvers.begin())) {
+        stage_dev_free(d);
+        d = *dev = new StageDev();
+        d->solvers.assign(ss, ss + n);
+        // the largest table is advect's; a step writes fewer than 24 tables
+        d->cap = (size_t)n * 24 * std::max<size_t>(sizeof(AdvectArgs), 256) + 24 * 256;
+        IFL_HIP(hipHostMalloc((void**)&d->arena_h, d->cap, hipHostMallocDefault));
+        IFL_HIP(hipMalloc((void**)&d->arena_d, d->cap));
+        IFL_HIP(hipMalloc((void**)&d->flags_d, 2 * n * sizeof(int)));
+        IFL_HIP(hipHostMalloc((void**)&d->flags_h, 2 * n * sizeof(int), hipHostMallocDefault));
+        d->ready.assign(n, nullptr);
+        for (int i = 0; i < n; i++) IFL_HIP(hipEventCreateWithFlags(&d->ready[i], hipEventDisableTiming));
+        d->maps.resize(MS_COUNT);
+    }
+    d->stream = stream;
+    // the previous step's table copies are complete before the arena is reused
+    IFL_HIP(hipStreamSynchronize(stream));
+    d->off = 0;
+    // work the caller queued on a solver's own stream comes first
+    for (int i = 0; i < n; i++) {
+        IFL_HIP(hipEventRecord(d->ready[i], ss[i]->stream));
+        IFL_HIP(hipStreamWaitEvent(stream, d->ready[i], 0));
+    }
+    return IFL_OK;
+}
+
+// k_fill_solid and (for d) k_update_dflag
+int kb_fill_solid(StageDev* d, int qi) {
+    Solver* s = d->solvers[0];
+    const int lv = s->level;
+    const FillArgs* tab;
+    IFL_TRY(stage_table<FillArgs>(d, [&](Solver* si, FillArgs& a) {
+        a.q = si->q[qi];
+        a.bodies = si->bodies_d;
+        a.nb = si->nbodies;
+        a.level = si->level;
+    }, &tab));
+    const StageMap* m;
+    if (lv >= 5) {
+        IFL_TRY(stage_map(d, MS_PHI + qi, per_solver(d, [&](Solver* si) {
+            return si->nbodies ? (long)(si->q[qi].w + 1) * (si->q[qi].h + 1) : 0L; }), &m));
+        IFL_BS_LAUNCH(bs_fill_phi_kernel, tab, m);
+    }
+    IFL_TRY(stage_map(d, MS_FILL + qi, per_solver(d, [&](Solver* si) {
+        return si->nbodies ? (long)si->q[qi].w * si->q[qi].h : 0L; }), &m));
+    IFL_BS_LAUNCH(bs_fill_cells_kernel, tab, m);
+    if (qi != IFL_Q_D) return IFL_OK;
+    const DflagArgs* ft;
+    IFL_TRY(stage_table<DflagArgs>(d, [&](Solver* si, DflagArgs& a) {
+        a.cell = si->q[IFL_Q_D].cell;
+        a.dflag = si->dflag;
+        a.fmask = si->fmask;
+        a.sk = si->sk;
+        a.n = (long)si->sk.nstrips * si->sk.strip_sz;
+        a.all_fluid = si->level == 3 ? 1 : 0;
+    }, &ft));
+    IFL_TRY(stage_map(d, MS_CELLS, per_solver(d, [](Solver* si) { return (long)si->W * si->H; }), &m));
+    IFL_BS_LAUNCH(bs_dflag_kernel, ft, m);
+    IFL_TRY(stage_map(d, MS_SKEW, per_solver(d, [](Solver* si) { return (long)si->sk.nstrips * si->sk.strip_sz; }),
+                      &m));
+    IFL_BS_LAUNCH(bs_fmask_kernel, ft, m);
+    return IFL_OK;
+}
+
+static long faces(const Solver* si) { return (long)(si->W + 1) * si->H + (long)si->W * (si->H + 1); }
+
+int kb_set_boundary(StageDev* d) {
+    Solver* s = d->solvers[0];
+    const BoundArgs* tab;
+    IFL_TRY(stage_table<BoundArgs>(d, [&](Solver* si, BoundArgs& a) {
+        a.u = si->q[IFL_Q_U].src;
+        a.v = si->q[IFL_Q_V].src;
+        a.cell = si->q[IFL_Q_D].cell;
+        a.body = si->q[IFL_Q_D].body;
+        a.bodies = si->bodies_d;
+        a.W = si->W;
+        a.H = si->H;
+        a.hx = si->hx;
+    }, &tab));
+    const StageMap* m;
+    IFL_TRY(stage_map(d, MS_FACES, per_solver(d, faces), &m));
+    IFL_BS_LAUNCH(bs_boundary_kernel, tab, m);
+    return IFL_OK;
+}
+
+int kb_build_rhs(StageDev* d) {
+    Solver* s = d->solvers[0];
+    const RhsArgs* tab;
+    IFL_TRY(stage_table<RhsArgs>(d, [&](Solver* si, RhsArgs& a) {
+        a.u = si->q[IFL_Q_U].src;
+        a.v = si->q[IFL_Q_V].src;
+        a.uvol = si->q[IFL_Q_U].volume;
+        a.vvol = si->q[IFL_Q_V].volume;
+        a.dvol = si->q[IFL_Q_D].volume;
+        a.cell = si->q[IFL_Q_D].cell;
+        a.body = si->q[IFL_Q_D].body;
+        a.r = si->r;
+        a.sk = si->sk;
+        a.level = si->level;
+        a.nbodies = si->nbodies;
+        a.hx = si->hx;
+        a.bodies = si->bodies_d;
+    }, &tab));
+    const StageMap* m;
+    IFL_TRY(stage_map(d, MS_CELLS, per_solver(d, [](Solver* si) { return (long)si->W * si->H; }), &m));
+    IFL_BS_LAUNCH(bs_rhs_kernel, tab, m);
+    return IFL_OK;
+}
+
+int kb_compute_densities(StageDev* d) {
+    Solver* s = d->solvers[0];
+    const DensArgs* tab;
+    IFL_TRY(stage_table<DensArgs>(d, [&](Solver* si, DensArgs& a) {
+        const double rho = si->cfg.density;
+        a.t = si->q[IFL_Q_T].src;
+        a.d = si->q[IFL_Q_D].src;
+        a.ud = si->udens;
+        a.vd = si->vdens;
+        a.W = si->W;
+        a.H = si->H;
+        a.rho_air = rho;
+        a.t_amb = si->cfg.t_amb;
+        a.alpha = (si->cfg.density_soot - rho) / rho;
+    }, &tab));
+    const StageMap* m;
+    IFL_TRY(stage_map(d, MS_FACES, per_solver(d, faces), &m));
+    IFL_BS_LAUNCH(bs_densities_kernel, tab, m);
+    return IFL_OK;
+}
+
+int kb_build_matrix(StageDev* d, int heat, double dt) {
+    Solver* s = d->solvers[0];
+    const MatArgs* tab;
+    IFL_TRY(stage_table<MatArgs>(d, [&](Solver* si, MatArgs& a) {
+        const double hx = si->hx;
+        a.cell = si->q[IFL_Q_D].cell;
+        a.uvol = si->q[IFL_Q_U].volume;
+        a.vvol = si->q[IFL_Q_V].volume;
+        a.ud = si->udens;
+        a.vd = si->vdens;
+        a.adiag = si->adiag;
+        a.apx = si->aplusx;
+        a.apy = si->aplusy;
+        a.sk = si->sk;
+        a.level = si->level;
+        a.heat = heat;
+        // k_build_heat / k_build_pressure
+        a.scale = heat ? si->cfg.diffusion * dt * 1.0 / (hx * hx)
+                       : si->level >= 7 ? dt / (hx * hx) : dt / (si->cfg.density * hx * hx);
+    }, &tab));
+    const StageMap* m;
+    IFL_TRY(stage_map(d, MS_CELLS, per_solver(d, [](Solver* si) { return (long)si->W * si->H; }), &m));
+    IFL_BS_LAUNCH(bs_matrix_kernel, tab, m);
+    return IFL_OK;
+}
+
+int kb_apply_pressure(StageDev* d, double dt) {
+    Solver* s = d->solvers[0];
+    const PressArgs* tab;
+    IFL_TRY(stage_table<PressArgs>(d, [&](Solver* si, PressArgs& a) {
+        a.u = si->q[IFL_Q_U].src;
+        a.v = si->q[IFL_Q_V].src;
+        a.p = si->p;
+        a.cell = si->q[IFL_Q_D].cell;
+        a.ud = si->udens;
+        a.vd = si->vdens;
+        a.sk = si->sk;
+        a.level = si->level;
+        a.scale = si->level >= 7 ? dt / si->hx : dt / (si->cfg.density * si->hx);
+    }, &tab));
+    const StageMap* m;
+    IFL_TRY(stage_map(d, MS_FACES, per_solver(d, faces), &m));
+    IFL_BS_LAUNCH(bs_pressure_kernel, tab, m);
+    return IFL_OK;
+}
+
+int kb_add_buoyancy(StageDev* d, double dt) {
+    Solver* s = d->solvers[0];
+    const BuoyArgs* tab;
+    IFL_TRY(stage_table<BuoyArgs>(d, [&](Solver* si, BuoyArgs& a) {
+        const double rho = si->cfg.density;
+        a.v = si->q[IFL_Q_V].src;
+        a.d = si->q[IFL_Q_D].src;
+        a.t = si->q[IFL_Q_T].src;
+        a.W = si->W;
+        a.H = si->H;
+        a.dtg = dt * si->cfg.gravity;
+        a.alpha = (si->cfg.density_soot - rho) / rho;
+        a.t_amb = si->cfg.t_amb;
+    }, &tab));
+    const StageMap* m;
+    IFL_TRY(stage_map(d, MS_VFACES, per_solver(d, [](Solver* si) { return (long)si->W * (si->H + 1); }), &m));
+    IFL_BS_LAUNCH(bs_buoyancy_kernel, tab, m);
+    return IFL_OK;
+}
+
+int kb_copy_t_to_r(StageDev* d) {
+    Solver* s = d->solvers[0];
+    const CopyArgs* tab;
+    IFL_TRY(stage_table<CopyArgs>(d, [&](Solver* si, CopyArgs& a) {
+        a.src = si->q[IFL_Q_T].src;
+        a.dst = si->r;
+        a.sk = si->sk;
+    }, &tab));
+    const StageMap* m;
+    IFL_TRY(stage_map(d, MS_CELLS, per_solver(d, [](Solver* si) { return (long)si->W * si->H; }), &m));
+    IFL_BS_LAUNCH(bs_rows_to_skew_kernel, tab, m);
+    return IFL_OK;
+}
+
+int kb_copy_p_to_t(StageDev* d) {
+    Solver* s = d->solvers[0];
+    const CopyArgs* tab;
+    IFL_TRY(stage_table<CopyArgs>(d, [&](Solver* si, CopyArgs& a) {
+        a.src = si->p;
+        a.dst = si->q[IFL_Q_T].src;
+        a.sk = si->sk;
+    }, &tab));
+    const StageMap* m;
+    IFL_TRY(stage_map(d, MS_CELLS, per_solver(d, [](Solver* si) { return (long)si->W * si->H; }), &m));
+    IFL_BS_LAUNCH(bs_skew_to_rows_kernel, tab, m);
+    return IFL_OK;
+}
+
+// k_advect with device-evaluated bodies in the FP64 form
+int kb_advect(StageDev* d, int qmask, double dt) {
+    Solver* s = d->solvers[0];
+    int qs[4], nq = 0;
+    for (int qi = 0; qi < 4; qi++)
+        if (qmask & (1 << qi)) qs[nq++] = qi;
+    if (nq == 0) return IFL_OK;
+    const AdvectArgs* tab;
+    IFL_TRY(stage_table<AdvectArgs>(d, [&](Solver* si, AdvectArgs& a) {
+        a.nq = nq;
+        for (int k = 0; k < nq; k++) a.q[k] = si->q[qs[k]];
+        const QtyView& u = si->q[IFL_Q_U];
+        const QtyView& v = si->q[IFL_Q_V];
+        a.u = u.src;
+        a.v = v.src;
+        a.uw = u.w; a.uh = u.h; a.vw = v.w; a.vh = v.h;
+        a.hx = si->hx;
+        a.dt = dt;
+        a.level = si->level;
+        a.bodies = si->bodies_d;
+        a.req = nullptr;
+        a.req_cnt = nullptr;
+        a.req_cap = 0;
+    }, &tab));
+    const int n = (int)d->solvers.size();
+    std::vector<long> c((size_t)nq * n);
+    for (int k = 0; k < nq; k++)
+        for (int i = 0; i < n; i++) c[(size_t)k * n + i] = (long)d->solvers[i]->q[qs[k]].w * d->solvers[i]->q[qs[k]].h;
+    const StageMap* m;
+    IFL_TRY(stage_map(d, MS_ADVECT + qmask, c, &m));
+    IFL_BS_LAUNCH(bs_advect_kernel, tab, m);
+    return IFL_OK;
+}
+
+int kb_finish(StageDev* d, bool border, int* nonfinite, int* border_solid) {
+    Solver* s = d->solvers[0];
+    const int n = (int)d->solvers.size();
+    int qs[4], nq = 0;
+    for (int qi = 0; qi < 4; qi++)
+        if (qi != IFL_Q_T || s->level >= 6) qs[nq++] = qi;
+    const FinishArgs* tab;
+    int i0 = 0;
+    IFL_TRY(stage_table<FinishArgs>(d, [&](Solver* si, FinishArgs& a) {
+        for (int k = 0; k < 4; k++) {
+            a.src[k] = k < nq ? si->q[qs[k]].src : nullptr;
+            a.n[k] = k < nq ? (long)si->q[qs[k]].w * si->q[qs[k]].h : 0;
+        }
+        a.flags = d->flags_d + 2 * i0++;
+        a.border = border ? si->front_cnt + 1 : nullptr;
+    }, &tab));
+    bs_flags_kernel<<<grid_blocks(n, 256), 256, 0, d->stream>>>(tab, n);
+    IFL_LAUNCH_CHECK();
+    std::vector<long> c((size_t)nq * n);
+    for (int k = 0; k < nq; k++)
+        for (int i = 0; i < n; i++) c[(size_t)k * n + i] = (long)d->solvers[i]->q[qs[k]].w * d->solvers[i]->q[qs[k]].h;
+    const StageMap* m;
+    IFL_TRY(stage_map(d, MS_NONFINITE, c, &m));
+    IFL_BS_LAUNCH(bs_nonfinite_kernel, tab, m);
+    IFL_HIP(hipMemcpyAsync(d->flags_h, d->flags_d, 2 * n * sizeof(int), hipMemcpyDeviceToHost, d->stream));
+    IFL_HIP(hipStreamSynchronize(d->stream));
+    for (int i = 0; i < n; i++) {
+        nonfinite[i] = d->flags_h[2 * i];
+        border_solid[i] = d->flags_h[2 * i + 1];
+    }
     return IFL_OK;
 }
 

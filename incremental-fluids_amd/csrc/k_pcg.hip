@@ -1273,6 +1273,26 @@ __global__ __launch_bounds__(256) void batch_fin_sigma_init_kernel(const BatchIn
     fin_sigma_body(b.v, b.nstrips, 0, 1, b.tol, b.limit);
 }
 
+// The exits of project() for every solver of the batch, as k_project does
+// them: converged, p += alpha s (7:869) and _z = A s; out of budget (with a
+// budget), the deferred s update and _z = A s at non-FLUID cells (levels 4+).
+__global__ __launch_bounds__(256) void batch_tail_kernel(const BatchInst* t, const int2* map) {
+    const int2 m = map[blockIdx.x];
+    const BatchInst& b = t[m.x];
+    const VecArgs& a = b.v;
+    const int status = a.st->status;
+    const long n = (long)a.sk.nstrips * a.sk.strip_sz;
+    const long i = (long)m.y * blockDim.x + threadIdx.x;
+    if (status == IFL_SOLVE_CONVERGED) {
+        vec_update_body(a, m.y, 2, 0);
+        if (i < n) a.z[i] = a.q[i];
+    } else if (status == IFL_SOLVE_BUDGET && b.limit > 0) {
+        vec_update_body(a, m.y, 0, 0);
+        int x, y;
+        if (a.pm && i < n && skew_cell(a.sk, i, x, y) && !a.flag[i]) a.z[i] = a.q[i];
+    }
+}
+
 __global__ void batch_gather_kernel(PcgState* const* st, int n, PcgState* out) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) out[i] = *st[i];
@@ -1484,7 +1504,7 @@ int batch_gather(Solver* s, BatchDev* d, hipStream_t stream) {
 // The batched project(): the entry (7:854-864) and the exit tails per solver,
 // exactly as k_project does them, and the iterations in between batched.
 int k_project_batch(Solver* const* ss, int n, hipStream_t stream, BatchDev** dev, ifl_solve_stats* stats,
-                    BatchProfile* prof) {
+                    BatchProfile* prof, bool join) {
     Solver* s0 = ss[0];
     const int lv = s0->level;
     if (s0->cfg.precision != IFL_PREC_FP64 || s0->cfg.dot_order != IFL_DOT_TREE) {
@@ -1508,8 +1528,10 @@ int k_project_batch(Solver* const* ss, int n, hipStream_t stream, BatchDev** dev
     int limit = 0;
     Solver* s = s0;  // error reporting
     for (int i = 0; i < n; i++) {
-        IFL_HIP(hipEventRecord(d->ready[i], ss[i]->stream));
-        IFL_HIP(hipStreamWaitEvent(stream, d->ready[i], 0));
+        if (join) {
+            IFL_HIP(hipEventRecord(d->ready[i], ss[i]->stream));
+            IFL_HIP(hipStreamWaitEvent(stream, d->ready[i], 0));
+        }
         limit = std::max(limit, ss[i]->cfg.pcg_limit);
     }
     IFL_TRY(batch_tables(s, d, stream, nullptr));  // every instance iterates at first
@@ -1595,6 +1617,21 @@ int k_project_batch(Solver* const* ss, int n, hipStream_t stream, BatchDev** dev
             prof->cell_iterations[IFL_KERNEL_SWEEP_FWD] += cells * f;
             prof->cell_iterations[IFL_KERNEL_SWEEP_BWD] += cells * b;
         }
+    }
+    if (!join) {
+        // every tail in one launch over the whole tables
+        IFL_TRY(batch_tables(s, d, stream, nullptr));
+        batch_tail_kernel<<<d->nvblk, 256, 0, stream>>>(d->inst, d->vmap);
+        IFL_LAUNCH_CHECK();
+        for (int i = 0; i < n; i++) {
+            const PcgState& h = d->states_host[i];
+            if (stats) {
+                stats[i].status = h.status;
+                stats[i].iterations = h.printed_iter;
+                stats[i].max_error = h.max_err;
+            }
+        }
+        return IFL_OK;
     }
     // the tails back on the solvers' streams, after the batched iterations
     IFL_HIP(hipEventRecord(d->joined, stream));
