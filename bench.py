@@ -485,13 +485,13 @@ def measure_batch(n_inst, steps, warmup, mode="fused", threads=16, level=4):
     if mode == "fused":
         batch = F.Batch(solvers)
 
+        inflows = [(k,) + tuple(inf) for k in range(len(insts)) for inf in insts[k]["inflows"]]
+
         def step_all(steps):  # noqa: F811 - the fused form replaces the threaded one
             iters = [0] * len(insts)
             t0 = time.perf_counter()
             for _ in range(steps):
-                for k in range(len(insts)):
-                    for inf in insts[k]["inflows"]:
-                        solvers[k].add_inflow(*inf)
+                batch.add_inflow(inflows)  # every instance's addInflow, one call
                 st = batch.update(insts[0]["dt"])
                 for k, x in enumerate(st):
                     iters[k] += x.pressure.iterations

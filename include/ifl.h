@@ -210,14 +210,25 @@ int ifl_init_particles(ifl_solver* s);
  * keep their own state; ifl_batch_update runs FluidSolver::update(dt) on all
  * of them (the reference steps each in turn, 7:1110-1126), with each PCG
  * iteration of their project() calls one launch per phase for the whole
- * batch. Level 8 solvers draw frand() in batch order within each step, as
- * the same solvers stepped one after the other would. stats receives n
- * entries. No counterpart in the reference: a caller with many small
- * simulations would otherwise step them one by one. */
+ * batch; at levels 3..7 with device-evaluated bodies every other stage of
+ * update() is one launch for the whole batch too. Level 8 solvers draw
+ * frand() in batch order within each step, as the same solvers stepped one
+ * after the other would. stats receives n entries. No counterpart in the
+ * reference: a caller with many small simulations would otherwise step them
+ * one by one. */
 typedef struct ifl_batch ifl_batch;
 int ifl_batch_create(ifl_solver* const* solvers, int n, ifl_batch** out);
 int ifl_batch_update(ifl_batch* batch, double dt, ifl_step_stats* stats);
 int ifl_batch_destroy(ifl_batch* batch);
+/* FluidQuantity::addInflow (3:167-184, 7:456-473) of many solvers of a batch in
+ * one call: entry k applies rect[4k..4k+3] = (x, y, w, h) with the values
+ * vals[4k..4k+3] (d, t, u, v) to the batch's solver inst[k] (an index into
+ * the list given to ifl_batch_create). A solver's entries apply in their
+ * order. The same as ifl_add_inflow on each entry in turn, with one launch
+ * per rank (every solver's first entry, then every second ...) and one
+ * wait. No counterpart in the reference, which calls addInflow per solver
+ * (7:1110-1126). */
+int ifl_batch_add_inflow(ifl_batch* batch, int count, const int* inst, const double* rect, const double* vals);
 /* Diagnostics, no counterpart in the reference: with profiling on, every
  * fused PCG launch of the batch (IFL_KERNEL_MATVEC, _SWEEP_FWD, _SWEEP_BWD)
  * is timed with events on the batch's stream. ifl_batch_profile(on) also

@@ -25,6 +25,7 @@ struct ifl_batch {
     ifl::BatchDev* dev = nullptr;
     ifl::ExtrapDev* xdev = nullptr;
     ifl::StageDev* sdev = nullptr;
+    ifl::InflowDev* idev = nullptr;
     bool profiling = false;
     ifl::BatchProfile prof;
 };
@@ -1122,6 +1123,23 @@ int ifl_batch_update(ifl_batch* b, double dt, ifl_step_stats* stats) {
     return rc;
 }
 
+int ifl_batch_add_inflow(ifl_batch* b, int count, const int* inst, const double* rect, const double* vals) {
+    if (!b || count < 0 || (count > 0 && (!inst || !rect || !vals))) return IFL_ERR_ARG;
+    const int n = (int)b->solvers.size();
+    for (int k = 0; k < count; k++)
+        if (inst[k] < 0 || inst[k] >= n) {
+            g_err = "ifl_batch_add_inflow: solver index out of range";
+            return IFL_ERR_ARG;
+        }
+    if (count == 0) return IFL_OK;
+    if (hipSetDevice(b->device) != hipSuccess) return IFL_ERR_HIP;
+    // work queued on the solvers' own streams comes first
+    for (Solver* s : b->solvers) IFL_HIP(hipStreamSynchronize(s->stream));
+    int rc = k_add_inflow_batch(b->solvers.data(), n, b->stream, &b->idev, count, inst, rect, vals);
+    if (rc != IFL_OK) g_err = b->solvers[0]->err;
+    return rc;
+}
+
 int ifl_batch_profile(ifl_batch* b, int on) {
     if (!b) return IFL_ERR_ARG;
     b->profiling = on != 0;
@@ -1147,6 +1165,7 @@ int ifl_batch_destroy(ifl_batch* b) {
     batch_dev_free(b->dev);
     extrap_dev_free(b->xdev);
     stage_dev_free(b->sdev);
+    inflow_dev_free(b->idev);
     delete b;
     return IFL_OK;
 }

@@ -245,6 +245,13 @@ int kb_advect(StageDev* d, int qmask, double dt);
 // the stream)
 int kb_finish(StageDev* d, bool border, int* nonfinite, int* border_solid);
 void stage_dev_free(StageDev* d);
+// addInflow of several solvers of a batch (entry k: solver ss[inst[k]], the
+// rectangle rect[4k..4k+3] = x, y, w, h and vals[4k..4k+3]); a solver's
+// entries apply in their order; waits for `stream`
+struct InflowDev;
+int k_add_inflow_batch(Solver* const* ss, int n, hipStream_t stream, InflowDev** dev, int count, const int* inst,
+                       const double* rect, const double* vals);
+void inflow_dev_free(InflowDev* d);
 // per-kernel totals of a batch's fused PCG launches (when profiling): every
 // launch timed with events on the batch's stream; cell_iterations counts the
 // cells of the instances that did work in them

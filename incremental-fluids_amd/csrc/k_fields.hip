@@ -764,42 +764,75 @@ int k_advect(Solver* s, int qmask, double dt) {
 }
 
 // ------------------------------------------------------------ addInflow 7:456-473
-__global__ void inflow_kernel(QtyView q, int xa, int xb, int ya, int yb, double x0, double y0, double x1, double y1,
-                              double v, int sequential) {
+struct InflowArgs {
+    QtyView q;
+    int xa, xb, ya, yb;  // the cell range
+    double x0, y0, x1, y1, v;
+    int sequential;
+};
+
+IFL_DEV void inflow_cell(const InflowArgs& a, int x, int y) {
+    const QtyView& q = a.q;
+    const double x0 = a.x0, y0 = a.y0, x1 = a.x1, y1 = a.y1;
     long n = (long)q.w * q.h;
-    int nx = xb - xa, ny = yb - ya;
-    auto body = [&](int x, int y) {
-        double l = length2((2.0 * (x + 0.5) * q.hx - (x0 + x1)) / (x1 - x0), (2.0 * (y + 0.5) * q.hx - (y0 + y1)) / (y1 - y0));
-        double vi = cubic_pulse(l) * v;
-        long idx = x + (long)y * q.w;
-        if (idx < 0 || idx >= n) return;  // the reference writes past its array here
-        if (fabs(q.src[idx]) < fabs(vi)) q.src[idx] = vi;
-    };
-    if (sequential) {  // x range wider than the row (Q3): rows alias, keep loop order
-        if (blockIdx.x == 0 && threadIdx.x == 0)
-            for (int y = ya; y < yb; y++)
-                for (int x = xa; x < xb; x++) body(x, y);
+    double l = length2((2.0 * (x + 0.5) * q.hx - (x0 + x1)) / (x1 - x0), (2.0 * (y + 0.5) * q.hx - (y0 + y1)) / (y1 - y0));
+    double vi = cubic_pulse(l) * a.v;
+    long idx = x + (long)y * q.w;
+    if (idx < 0 || idx >= n) return;  // the reference writes past its array here
+    if (fabs(q.src[idx]) < fabs(vi)) q.src[idx] = vi;
+}
+
+IFL_DEV void inflow_body(const InflowArgs& a, long gi) {
+    if (a.sequential) {  // x range wider than the row (Q3): rows alias, keep loop order
+        if (gi == 0)
+            for (int y = a.ya; y < a.yb; y++)
+                for (int x = a.xa; x < a.xb; x++) inflow_cell(a, x, y);
         return;
     }
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= nx * ny) return;
-    body(xa + i % nx, ya + i / nx);
+    const int nx = a.xb - a.xa, ny = a.yb - a.ya;
+    if (gi >= (long)nx * ny) return;
+    const int i = (int)gi;
+    inflow_cell(a, a.xa + i % nx, a.ya + i / nx);
+}
+
+__global__ void inflow_kernel(InflowArgs a) { inflow_body(a, blockIdx.x * (long)blockDim.x + threadIdx.x); }
+
+// the launch of quantity qi's inflow (7:456-473); false when its cell range
+// is empty
+static bool inflow_args(const Solver* s, int qi, double x0, double y0, double x1, double y1, double v,
+                        InflowArgs& a) {
+    const QtyView& q = s->q[qi];
+    int ix0 = (int)(x0 / q.hx - q.ox);
+    int iy0 = (int)(y0 / q.hx - q.oy);
+    int ix1 = (int)(x1 / q.hx - q.ox);
+    int iy1 = (int)(y1 / q.hx - q.oy);
+    int ya = std::max(iy0, 0), yb = std::min(iy1, q.h);
+    int xa = std::max(ix0, 0), xb = std::min(ix1, q.h);  // x bounded by _h, Q3 (7:463)
+    if (yb <= ya || xb <= xa) return false;
+    a.q = q;
+    a.xa = xa;
+    a.xb = xb;
+    a.ya = ya;
+    a.yb = yb;
+    a.x0 = x0;
+    a.y0 = y0;
+    a.x1 = x1;
+    a.y1 = y1;
+    a.v = v;
+    a.sequential = xb > q.w ? 1 : 0;
+    return true;
+}
+
+static long inflow_threads(const InflowArgs& a) {
+    return a.sequential ? 1 : (long)(a.xb - a.xa) * (a.yb - a.ya);
 }
 
 int k_add_inflow(Solver* s, double x0, double y0, double x1, double y1, const double vals[4]) {
     for (int qi = 0; qi < 4; qi++) {
         if (qi == IFL_Q_T && s->level < 6) continue;
-        QtyView q = s->q[qi];
-        int ix0 = (int)(x0 / q.hx - q.ox);
-        int iy0 = (int)(y0 / q.hx - q.oy);
-        int ix1 = (int)(x1 / q.hx - q.ox);
-        int iy1 = (int)(y1 / q.hx - q.oy);
-        int ya = std::max(iy0, 0), yb = std::min(iy1, q.h);
-        int xa = std::max(ix0, 0), xb = std::min(ix1, q.h);  // x bounded by _h, Q3 (7:463)
-        if (yb <= ya || xb <= xa) continue;
-        int seq = xb > q.w ? 1 : 0;
-        long cnt = (long)(xb - xa) * (yb - ya);
-        inflow_kernel<<<seq ? 1 : grid_blocks(cnt, 256), 256, 0, s->stream>>>(q, xa, xb, ya, yb, x0, y0, x1, y1, vals[qi], seq);
+        InflowArgs a;
+        if (!inflow_args(s, qi, x0, y0, x1, y1, vals[qi], a)) continue;
+        inflow_kernel<<<grid_blocks(inflow_threads(a), 256), 256, 0, s->stream>>>(a);
         IFL_LAUNCH_CHECK();
     }
     return IFL_OK;
@@ -1720,6 +1753,100 @@ int kb_finish(StageDev* d, bool border, int* nonfinite, int* border_solid) {
         nonfinite[i] = d->flags_h[2 * i];
         border_solid[i] = d->flags_h[2 * i + 1];
     }
+    return IFL_OK;
+}
+
+// ------------------------------------------------------------ batched addInflow
+// addInflow (7:456-473) of many solvers at once: one launch per rank (every
+// solver's first inflow, then every second one ...), a row per (inflow,
+// quantity) in the table, so a solver's inflows apply in their order.
+IFL_BATCH_STAGE(bs_inflow_kernel, InflowArgs, inflow_body(a, gi))
+
+struct InflowDev {
+    InflowArgs* rows_h = nullptr;  // pinned
+    InflowArgs* rows_d = nullptr;
+    int2* map_h = nullptr;         // pinned
+    int2* map_d = nullptr;
+    size_t rows_cap = 0, map_cap = 0;
+};
+
+void inflow_dev_free(InflowDev* d) {
+    if (!d) return;
+    if (d->rows_h) (void)hipHostFree(d->rows_h);
+    if (d->rows_d) (void)hipFree(d->rows_d);
+    if (d->map_h) (void)hipHostFree(d->map_h);
+    if (d->map_d) (void)hipFree(d->map_d);
+    delete d;
+}
+
+int k_add_inflow_batch(Solver* const* ss, int n, hipStream_t stream, InflowDev** dev, int count, const int* inst,
+                       const double* rect, const double* vals) {
+    Solver* s = ss[0];  // error reporting
+    if (!*dev) *dev = new InflowDev();
+    InflowDev* d = *dev;
+    std::vector<int> rank(count), seen(n, 0);
+    int ranks = 0;
+    for (int k = 0; k < count; k++) {
+        rank[k] = seen[inst[k]]++;
+        ranks = std::max(ranks, rank[k] + 1);
+    }
+    // rows and blocks, rank by rank
+    std::vector<InflowArgs> rows;
+    std::vector<int2> map;
+    std::vector<size_t> row0(ranks + 1), blk0(ranks + 1);
+    for (int r = 0; r < ranks; r++) {
+        row0[r] = rows.size();
+        blk0[r] = map.size();
+        for (int k = 0; k < count; k++) {
+            if (rank[k] != r) continue;
+            const Solver* si = ss[inst[k]];
+            const double* rc = rect + 4L * k;
+            const double x0 = rc[0], y0 = rc[1], x1 = rc[0] + rc[2], y1 = rc[1] + rc[3];  // ifl_add_inflow
+            for (int qi = 0; qi < 4; qi++) {
+                if (qi == IFL_Q_T && si->level < 6) continue;
+                InflowArgs a;
+                if (!inflow_args(si, qi, x0, y0, x1, y1, vals[4L * k + qi], a)) continue;
+                const int row = (int)(rows.size() - row0[r]);
+                for (long b = 0; b < grid_blocks(inflow_threads(a), 256); b++) map.push_back(make_int2(row, (int)b));
+                rows.push_back(a);
+            }
+        }
+    }
+    row0[ranks] = rows.size();
+    blk0[ranks] = map.size();
+    if (rows.size() > d->rows_cap) {
+        if (d->rows_h) IFL_HIP(hipHostFree(d->rows_h));
+        if (d->rows_d) IFL_HIP(hipFree(d->rows_d));
+        d->rows_h = nullptr;
+        d->rows_d = nullptr;
+        d->rows_cap = 0;
+        IFL_HIP(hipHostMalloc((void**)&d->rows_h, rows.size() * sizeof(InflowArgs), hipHostMallocDefault));
+        IFL_HIP(hipMalloc((void**)&d->rows_d, rows.size() * sizeof(InflowArgs)));
+        d->rows_cap = rows.size();
+    }
+    if (map.size() > d->map_cap) {
+        if (d->map_h) IFL_HIP(hipHostFree(d->map_h));
+        if (d->map_d) IFL_HIP(hipFree(d->map_d));
+        d->map_h = nullptr;
+        d->map_d = nullptr;
+        d->map_cap = 0;
+        IFL_HIP(hipHostMalloc((void**)&d->map_h, map.size() * sizeof(int2), hipHostMallocDefault));
+        IFL_HIP(hipMalloc((void**)&d->map_d, map.size() * sizeof(int2)));
+        d->map_cap = map.size();
+    }
+    if (rows.empty() || map.empty()) return IFL_OK;
+    std::copy(rows.begin(), rows.end(), d->rows_h);
+    std::copy(map.begin(), map.end(), d->map_h);
+    IFL_HIP(hipMemcpyAsync(d->rows_d, d->rows_h, rows.size() * sizeof(InflowArgs), hipMemcpyHostToDevice, stream));
+    IFL_HIP(hipMemcpyAsync(d->map_d, d->map_h, map.size() * sizeof(int2), hipMemcpyHostToDevice, stream));
+    for (int r = 0; r < ranks; r++) {
+        const int nblk = (int)(blk0[r + 1] - blk0[r]);
+        if (nblk == 0) continue;
+        bs_inflow_kernel<<<nblk, 256, 0, stream>>>(d->rows_d + row0[r], d->map_d + blk0[r]);
+        IFL_LAUNCH_CHECK();
+    }
+    // the tables are rewritten by the next call
+    IFL_HIP(hipStreamSynchronize(stream));
     return IFL_OK;
 }
 

@@ -104,6 +104,8 @@ def _load():
     lib.ifl_batch_create.argtypes = [ctypes.POINTER(vp), ctypes.c_int, ctypes.POINTER(vp)]
     lib.ifl_batch_update.argtypes = [vp, ctypes.c_double, ctypes.POINTER(StepStats)]
     lib.ifl_batch_destroy.argtypes = [vp]
+    dp = ctypes.POINTER(ctypes.c_double)
+    lib.ifl_batch_add_inflow.argtypes = [vp, ctypes.c_int, ctypes.POINTER(ctypes.c_int), dp, dp]
     lib.ifl_batch_profile.argtypes = [vp, ctypes.c_int]
     lib.ifl_batch_profile_read.argtypes = [vp, ctypes.c_int, ctypes.POINTER(ctypes.c_int64),
                                            ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double)]
@@ -197,9 +199,15 @@ class FluidSolver:
             raise IflError(f"ifl call failed rc={rc}: {msg.decode() if msg else ''}")
 
     def _push_bodies(self):
-        n = len(self.bodies)
-        arr = (BodyC * max(1, n))(*[b.to_c() for b in self.bodies])
-        self._check(lib.ifl_set_bodies(self._h, arr, n))
+        # the bodies are re-read every step (they may have moved); an
+        # unchanged list is not sent again
+        cs = [b.to_c() for b in self.bodies]
+        key = [bytes(c) for c in cs]
+        if key == getattr(self, "_pushed", None):
+            return
+        arr = (BodyC * max(1, len(cs)))(*cs)
+        self._check(lib.ifl_set_bodies(self._h, arr, len(cs)))
+        self._pushed = key
 
     def close(self):
         if self._h:
@@ -361,6 +369,21 @@ class Batch:
         rc = lib.ifl_batch_create(arr, len(self.solvers), ctypes.byref(self._h))
         if rc != 0:
             raise IflError(f"ifl_batch_create failed rc={rc}: {lib.ifl_last_error(None).decode()}")
+
+    def add_inflow(self, entries):
+        """addInflow of many solvers in one call: entries are (index into
+        the batch's solvers, x, y, w, h, d, t, u, v); a solver's entries
+        apply in their order (ifl_batch_add_inflow)."""
+        entries = list(entries)
+        if not entries:
+            return
+        k = len(entries)
+        inst = (ctypes.c_int * k)(*[int(e[0]) for e in entries])
+        rect = (ctypes.c_double * (4 * k))(*[float(v) for e in entries for v in e[1:5]])
+        vals = (ctypes.c_double * (4 * k))(*[float(v) for e in entries for v in e[5:9]])
+        rc = lib.ifl_batch_add_inflow(self._h, k, inst, rect, vals)
+        if rc != 0:
+            raise IflError(f"ifl_batch_add_inflow failed rc={rc}: {lib.ifl_last_error(None).decode()}")
 
     def update(self, dt):
         for s in self.solvers:

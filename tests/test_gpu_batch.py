@@ -67,6 +67,13 @@ def test_fused_batch_equals_single_solvers(level):
     params = (0.1,) if level <= 5 else (0.1, 1.0, 0.01)
     inflow = lambda i: i["inflows"][0] if level <= 5 else i["inflows"][0][:5] + (600.0, 0.0, 0.0)
 
+    def inflows(i):
+        # the second entry negates d over the same rectangle: addInflow keeps
+        # the value of larger magnitude, the first on a tie, so the entries'
+        # order decides the result
+        e = tuple(inflow(i))
+        return [e, e[:4] + (-e[4],) + e[5:]]
+
     def make(i):
         bodies = [] if level == 3 else [F.SolidBox(*b[1:]) for b in i["bodies"]]
         return F.FluidSolver(level, i["w"], i["h"], params, bodies)
@@ -75,8 +82,7 @@ def test_fused_batch_equals_single_solvers(level):
     batch = F.Batch(batched)
     iters_b, stat_b = [], []
     for _ in range(steps):
-        for s, i in zip(batched, insts):
-            s.add_inflow(*inflow(i))
+        batch.add_inflow([(k,) + e for k, i in enumerate(insts) for e in inflows(i)])
         st = batch.update(insts[0]["dt"])
         iters_b.append([(x.heat.iterations, x.pressure.iterations) for x in st])
         stat_b.append([(x.heat.status, x.pressure.status, x.border_solid, x.nonfinite) for x in st])
@@ -86,7 +92,8 @@ def test_fused_batch_equals_single_solvers(level):
         single = make(i)
         its, stat = [], []
         for _ in range(steps):
-            single.add_inflow(*inflow(i))
+            for e in inflows(i):
+                single.add_inflow(*e)
             st = single.update(i["dt"])
             its.append((st.heat.iterations, st.pressure.iterations))
             stat.append((st.heat.status, st.pressure.status, st.border_solid, st.nonfinite))
