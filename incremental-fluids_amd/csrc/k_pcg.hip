@@ -1184,6 +1184,11 @@ struct BatchDev {
     // the full block tables; the device tables hold those of the instances
     // still iterating (rebuilt when some have finished)
     std::vector<int2> vmap_h, fmap_h, bmap_h, vmap_c, fmap_c, bmap_c;
+    // the matrix-vector product and the vector update take kVecBlocks
+    // consecutive blocks of an instance per workgroup: (instance, group)
+    std::vector<int2> vmap4_h, vmap4_c;
+    int2* vmap4 = nullptr;
+    int nvblk4 = 0;
     int nvblk_full = 0, nsblk_full = 0;
     bool compacted = false;
     // profiling: events around the fused launches (matvec, forward and
@@ -1196,6 +1201,7 @@ void batch_dev_free(BatchDev* d) {
     if (!d) return;
     auto F = [](void* p) { if (p) (void)hipFree(p); };
     F(d->inst); F(d->fwd); F(d->bwd); F(d->pre); F(d->fwd0); F(d->bwd0); F(d->vmap); F(d->fmap); F(d->bmap); F(d->states); F(d->st_ptrs);
+    F(d->vmap4);
     if (d->states_host) (void)hipHostFree(d->states_host);
     for (hipEvent_t e : d->ready) (void)hipEventDestroy(e);
     for (hipEvent_t e : d->ev_pool) (void)hipEventDestroy(e);
@@ -1209,14 +1215,32 @@ void batch_dev_free(BatchDev* d) {
 
 namespace {
 
+// blocks of 256 cells per workgroup in the batch's vector update and
+// matrix-vector product: an instance's arguments are loaded once for all of
+// them (each block's partial is the single-solver kernel's)
+constexpr int kVecBlocks = 4;
+
 __global__ __launch_bounds__(256) void batch_vec_update_kernel(const BatchInst* t, const int2* map) {
     const int2 m = map[blockIdx.x];
-    vec_update_body(t[m.x].v, m.y, 0, 1);
+    const VecArgs v = t[m.x].v;
+    const int nbv = t[m.x].nbv;
+#pragma unroll
+    for (int k = 0; k < kVecBlocks; k++) {
+        const int blk = m.y * kVecBlocks + k;
+        if (blk < nbv) vec_update_body(v, blk, 0, 1);
+    }
 }
 
 __global__ __launch_bounds__(256) void batch_matvec_kernel(const BatchInst* t, const int2* map) {
     const int2 m = map[blockIdx.x];
-    matvec_body<1>(t[m.x].v, m.y, 1);
+    const VecArgs v = t[m.x].v;
+    const int nbv = t[m.x].nbv;
+    for (int k = 0; k < kVecBlocks; k++) {
+        const int blk = m.y * kVecBlocks + k;
+        if (blk >= nbv) break;  // the same for the whole workgroup
+        matvec_body<1>(v, blk, 1);
+        __syncthreads();  // the block reduction's LDS is reused by the next block
+    }
 }
 
 __global__ __launch_bounds__(256) void batch_fin_alpha_kernel(const BatchInst* t) {
@@ -1357,6 +1381,7 @@ int batch_build(Solver* const* ss, int n, BatchDev* d) {
         IFL_TRY((ring_setup<SW_BWD, 0, LV, NGB, double>(s, bwd0[i], &shb)));
         d->shmem_b0 = shb;
         for (int k = 0; k < b.nbv; k++) vmap.push_back(make_int2(i, k));
+        for (int k = 0; k * kVecBlocks < b.nbv; k++) d->vmap4_h.push_back(make_int2(i, k));
         sts[i] = s->st;
     }
     // Strips are dealt by their distance from the sweep's start (every
@@ -1397,6 +1422,8 @@ int batch_build(Solver* const* ss, int n, BatchDev* d) {
     IFL_TRY(upload(s, &d->fwd0, fwd0));
     IFL_TRY(upload(s, &d->bwd0, bwd0));
     IFL_TRY(upload(s, &d->vmap, vmap));
+    IFL_TRY(upload(s, &d->vmap4, d->vmap4_h));
+    d->nvblk4 = (int)d->vmap4_h.size();
     IFL_TRY(upload(s, &d->fmap, fmap));
     IFL_TRY(upload(s, &d->bmap, bmap));
     IFL_TRY(upload(s, &d->st_ptrs, sts));
@@ -1438,11 +1465,11 @@ int batch_time(Solver* s, BatchDev* d, hipStream_t stream, int kernel, bool begi
 template <int LV, int NGF, int NGB>
 int batch_iteration(Solver* s, BatchDev* d, hipStream_t stream, int it, bool prof) {
     if (it > 0) {
-        batch_vec_update_kernel<<<d->nvblk, 256, 0, stream>>>(d->inst, d->vmap);
+        batch_vec_update_kernel<<<d->nvblk4, 256, 0, stream>>>(d->inst, d->vmap4);
         IFL_LAUNCH_CHECK();
     }
     if (prof) IFL_TRY(batch_time(s, d, stream, IFL_KERNEL_MATVEC, true));
-    batch_matvec_kernel<<<d->nvblk, 256, 0, stream>>>(d->inst, d->vmap);
+    batch_matvec_kernel<<<d->nvblk4, 256, 0, stream>>>(d->inst, d->vmap4);
     IFL_LAUNCH_CHECK();
     if (prof) IFL_TRY(batch_time(s, d, stream, IFL_KERNEL_MATVEC, false));
     batch_fin_alpha_kernel<<<d->n, 256, 0, stream>>>(d->inst);
@@ -1469,6 +1496,7 @@ int batch_tables(Solver* s, BatchDev* d, hipStream_t stream, const std::vector<c
     if (!active) {
         if (!d->compacted) return IFL_OK;
         d->vmap_c = d->vmap_h;
+        d->vmap4_c = d->vmap4_h;
         d->fmap_c = d->fmap_h;
         d->bmap_c = d->bmap_h;
     } else {
@@ -1478,12 +1506,16 @@ int batch_tables(Solver* s, BatchDev* d, hipStream_t stream, const std::vector<c
                 if ((*active)[m.x]) out.push_back(m);
         };
         pick(d->vmap_h, d->vmap_c);
+        pick(d->vmap4_h, d->vmap4_c);
         pick(d->fmap_h, d->fmap_c);
         pick(d->bmap_h, d->bmap_c);
     }
     d->nvblk = (int)d->vmap_c.size();
+    d->nvblk4 = (int)d->vmap4_c.size();
     d->nsblk = (int)d->fmap_c.size();
     IFL_HIP(hipMemcpyAsync(d->vmap, d->vmap_c.data(), d->vmap_c.size() * sizeof(int2), hipMemcpyHostToDevice, stream));
+    IFL_HIP(hipMemcpyAsync(d->vmap4, d->vmap4_c.data(), d->vmap4_c.size() * sizeof(int2), hipMemcpyHostToDevice,
+                           stream));
     IFL_HIP(hipMemcpyAsync(d->fmap, d->fmap_c.data(), d->fmap_c.size() * sizeof(int2), hipMemcpyHostToDevice, stream));
     IFL_HIP(hipMemcpyAsync(d->bmap, d->bmap_c.data(), d->bmap_c.size() * sizeof(int2), hipMemcpyHostToDevice, stream));
     IFL_HIP(hipStreamSynchronize(stream));  // the host tables are rebuilt in place next time
