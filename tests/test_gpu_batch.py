@@ -59,7 +59,9 @@ def test_fused_batch_equals_single_solvers(level):
     must end exactly where the same instance stepped on its own ends (the
     same kernel bodies on the same arrays, the same reduction order per
     instance), while the instances converge after different iteration
-    counts."""
+    counts. Instance 1 has no body (its fill tables are empty); from level 5
+    on the others' boxes move and turn, advanced between steps as the
+    reference's main loop does."""
     import numpy as np
     import ifl_fluids as F
     import ifl_scenes as S
@@ -74,27 +76,35 @@ def test_fused_batch_equals_single_solvers(level):
         e = tuple(inflow(i))
         return [e, e[:4] + (-e[4],) + e[5:]]
 
-    def make(i):
-        bodies = [] if level == 3 else [F.SolidBox(*b[1:]) for b in i["bodies"]]
+    def make(k, i):
+        if level == 3 or k == 1:
+            bodies = []
+        else:
+            # (x, y, sx, sy, theta, vx, vy, vtheta); moving from level 5 on
+            vel = (0.08, -0.05, 0.6) if level >= 5 else (0.0, 0.0, 0.0)
+            bodies = [F.SolidBox(*(tuple(b[1:6]) + vel)) for b in i["bodies"]]
         return F.FluidSolver(level, i["w"], i["h"], params, bodies)
     steps = 3
-    batched = [make(i) for i in insts]
+    batched = [make(k, i) for k, i in enumerate(insts)]
     batch = F.Batch(batched)
     iters_b, stat_b = [], []
     for _ in range(steps):
         batch.add_inflow([(k,) + e for k, i in enumerate(insts) for e in inflows(i)])
         st = batch.update(insts[0]["dt"])
+        for s in batched:
+            s.bodies_update(insts[0]["dt"])
         iters_b.append([(x.heat.iterations, x.pressure.iterations) for x in st])
         stat_b.append([(x.heat.status, x.pressure.status, x.border_solid, x.nonfinite) for x in st])
     batch.close()
     fields = ol.state_fields(level)
     for k, i in enumerate(insts):
-        single = make(i)
+        single = make(k, i)
         its, stat = [], []
         for _ in range(steps):
             for e in inflows(i):
                 single.add_inflow(*e)
             st = single.update(i["dt"])
+            single.bodies_update(i["dt"])
             its.append((st.heat.iterations, st.pressure.iterations))
             stat.append((st.heat.status, st.pressure.status, st.border_solid, st.nonfinite))
         assert its == [row[k] for row in iters_b]
