@@ -428,25 +428,33 @@ int fill_solid(Solver* s, int qi) {
 int update_part(Solver* s, int after, double dt, int* next) {
     const int lv = s->level;
     *next = SOLVE_NONE;
-    if (after == SOLVE_NONE) {
-        s->border_solid = 0;
-        s->border_pending = false;
-        if (lv == 3) {
-            IFL_TRY(k_build_rhs(s));
-            IFL_TRY(k_build_pressure(s, dt));
-            if (!s->batched) IFL_TRY(k_build_precon(s));
-            *next = SOLVE_PRESSURE;
-            return IFL_OK;
-        }
-        if (lv == 8 && !s->particles_ready) IFL_TRY(flip_init_particles(s));
-        IFL_TRY(fill_solid(s, IFL_Q_D));
-        if (lv >= 6) IFL_TRY(fill_solid(s, IFL_Q_T));
-        IFL_TRY(fill_solid(s, IFL_Q_U));
-        IFL_TRY(fill_solid(s, IFL_Q_V));
-        if (lv == 8) {
+    if (after == SOLVE_NONE || after == STAGE_PRUNED) {
+        if (after == SOLVE_NONE) {
+            s->border_solid = 0;
+            s->border_pending = false;
+            if (lv == 3) {
+                IFL_TRY(k_build_rhs(s));
+                IFL_TRY(k_build_pressure(s, dt));
+                if (!s->batched) IFL_TRY(k_build_precon(s));
+                *next = SOLVE_PRESSURE;
+                return IFL_OK;
+            }
+            if (lv == 8 && !s->particles_ready) IFL_TRY(flip_init_particles(s));
+            IFL_TRY(fill_solid(s, IFL_Q_D));
+            if (lv >= 6) IFL_TRY(fill_solid(s, IFL_Q_T));
+            IFL_TRY(fill_solid(s, IFL_Q_U));
+            IFL_TRY(fill_solid(s, IFL_Q_V));
+            if (lv == 8 && s->flip_split) {
+                // the batch reads every solver's prune and seeds them in order
+                IFL_TRY(k_particles_to_grid_begin(s));
+                *next = STAGE_PRUNED;
+                return IFL_OK;
+            }
             // 8:1357-1368: particles to grid, the pre-update copies, and the
             // inflow the FLIP solver applies itself
-            IFL_TRY(k_particles_to_grid(s));
+            if (lv == 8) IFL_TRY(k_particles_to_grid(s));
+        }
+        if (lv == 8) {
             IFL_TRY(k_update_dflag(s));
             if (s->cfg.verbose) printf("Particle count: %d\n", s->particle_count);
             for (int qi = 0; qi < 4; qi++) IFL_TRY(k_copy_old(s, qi));
@@ -502,7 +510,7 @@ int update_part(Solver* s, int after, double dt, int* next) {
         IFL_TRY(k_grid_to_particles(s, alpha));
         for (int qi = 0; qi < 4; qi++) IFL_TRY(k_diff(s, qi, alpha, 1));
         IFL_TRY(k_particle_advect(s, dt));
-        IFL_TRY(flip_check_overflow(s));
+        if (!s->flip_split) IFL_TRY(flip_check_overflow(s));  // else at the batch's step end
     } else {
         int qm = (1 << IFL_Q_D) | (1 << IFL_Q_U) | (1 << IFL_Q_V) | (lv >= 6 ? (1 << IFL_Q_T) : 0);
         IFL_TRY(k_advect(s, qm, dt));
@@ -1064,6 +1072,21 @@ int ifl_batch_update(ifl_batch* b, double dt, ifl_step_stats* stats) {
     // streams, joined to the batched solves by events.
     bool staged = b->solvers[0]->level <= 7;
     for (Solver* s : b->solvers) staged = staged && !s->host_geom;
+    // Level 8 once every solver has its particles (creation draws frand()
+    // in step order): particlesToGrid runs for every solver up to its first
+    // host wait, then the batch reads the prunes' counts and seeds the
+    // solvers in batch order (the draws of the same solvers stepped one
+    // after the other).
+    struct Split {
+        std::vector<Solver*>& v;
+        explicit Split(std::vector<Solver*>& x, bool on) : v(x) { for (Solver* s : v) s->flip_split = on; }
+        ~Split() { for (Solver* s : v) s->flip_split = false; }
+    } split(b->solvers, [&] {
+        bool on = b->solvers[0]->level == 8;
+        for (Solver* s : b->solvers) on = on && s->particles_ready;
+        return on;
+    }());
+    const bool flip_split = b->solvers[0]->flip_split;
     if (staged) {
         rc = kb_begin(b->solvers.data(), n, b->stream, &b->sdev);
         if (rc != IFL_OK) g_err = b->solvers[0]->err;
@@ -1082,6 +1105,18 @@ int ifl_batch_update(ifl_batch* b, double dt, ifl_step_stats* stats) {
             }
         }
         if (rc != IFL_OK || next == SOLVE_NONE) break;
+        if (next == STAGE_PRUNED) {
+            for (int i = 0; i < n && rc == IFL_OK; i++) {
+                rc = k_prune_finish(b->solvers[i]);
+                if (rc != IFL_OK) g_err = b->solvers[i]->err;
+            }
+            for (int i = 0; i < n && rc == IFL_OK; i++) {
+                rc = k_seed(b->solvers[i]);
+                if (rc != IFL_OK) g_err = b->solvers[i]->err;
+            }
+            after = next;
+            continue;
+        }
         if (next == STAGE_EXTRAPOLATE)
             rc = k_extrapolate_batch(b->solvers.data(), n, b->stream, &b->xdev,
                                      (1 << IFL_Q_D) | (1 << IFL_Q_U) | (1 << IFL_Q_V), !staged);
@@ -1114,6 +1149,10 @@ int ifl_batch_update(ifl_batch* b, double dt, ifl_step_stats* stats) {
                 stats[i].border_solid = s->border_solid;
             }
         }
+    }
+    for (int i = 0; i < n && rc == IFL_OK && flip_split; i++) {
+        rc = flip_check_overflow(b->solvers[i]);
+        if (rc != IFL_OK) g_err = b->solvers[i]->err;
     }
     for (int i = 0; i < n && rc == IFL_OK && !staged; i++) {
         rc = finish_step(b->solvers[i], heat[i], press[i], stats ? &stats[i] : nullptr);

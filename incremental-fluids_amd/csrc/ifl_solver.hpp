@@ -147,6 +147,11 @@ struct Solver {
     int num_cus = 1;           // compute units: the strips of one sweep launch that can be resident at once
     std::atomic<int>* busy = nullptr;  // calls stepping solvers on this device right now
     bool batched = false;      // stepped by ifl_batch_update: the batch builds the MIC(0) factors
+    // level 8 in a batch: particlesToGrid stops before its first host wait
+    // (the prune's count) and the batch finishes every solver's prune, then
+    // seeds them in batch order; the overflow check moves to the step's end
+    bool flip_split = false;
+    bool prune_pending = false;  // a prune's count is still to be read
 };
 
 // error helpers
@@ -205,7 +210,8 @@ int k_rows_to_skew(Solver* s, const double* rows, double* skew);
 
 // the linear solves of one update() (FluidSolver::project calls)
 enum { SOLVE_NONE = 0, SOLVE_HEAT = 1, SOLVE_PRESSURE = 2,
-       STAGE_EXTRAPOLATE = 3 };  // batched solvers (4-7): d, u, v extrapolated by the batch
+       STAGE_EXTRAPOLATE = 3,  // batched solvers (4-7): d, u, v extrapolated by the batch
+       STAGE_PRUNED = 4 };     // batched FLIP solvers: pruned on the device, the batch reads and seeds
 
 // ---- launchers (k_pcg.hip) ----
 int k_build_precon(Solver* s);
@@ -288,6 +294,11 @@ int k_from_particles(Solver* s, int qi, bool with_t);
 int k_extrapolate_flip(Solver* s, int qi);
 int k_grid_to_particles(Solver* s, double alpha);
 int k_particles_to_grid(Solver* s);
+// particlesToGrid up to the prune's kernels (no host wait), and the prune's
+// count read back: k_particles_to_grid_begin, k_prune_finish, k_seed together
+// are k_particles_to_grid
+int k_particles_to_grid_begin(Solver* s);
+int k_prune_finish(Solver* s);
 int k_particle_advect(Solver* s, double dt);
 int k_copy_old(Solver* s, int qi);
 int k_diff(Solver* s, int qi, double alpha, int undo);

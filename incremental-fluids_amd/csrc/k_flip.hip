@@ -1187,12 +1187,22 @@ static int prune(Solver* s) {
     IFL_LAUNCH_CHECK();
     prune_apply_kernel<<<256, 256, 0, s->stream>>>(pa, s->pidx2, (const int*)s->pkey, dev_sc(s));
     IFL_LAUNCH_CHECK();
+    s->prune_pending = true;
+    return IFL_OK;
+}
+
+int k_prune_finish(Solver* s) {
+    if (!s->prune_pending) return IFL_OK;
+    s->prune_pending = false;
     IFL_TRY(read_scalars(s));
     s->particle_count = host_sc(s)->count;
     return IFL_OK;
 }
 
-int k_prune(Solver* s) { return fp32_particles(s) ? prune<float>(s) : prune<double>(s); }
+int k_prune(Solver* s) {
+    IFL_TRY(fp32_particles(s) ? prune<float>(s) : prune<double>(s));
+    return k_prune_finish(s);
+}
 
 template <typename P>
 static int seed(Solver* s) {
@@ -1350,8 +1360,8 @@ int k_extrapolate_flip(Solver* s, int qi) {
     return IFL_OK;
 }
 
-// particlesToGrid 8:916-927
-int k_particles_to_grid(Solver* s) {
+// particlesToGrid 8:916-927 up to the prune's kernels
+int k_particles_to_grid_begin(Solver* s) {
     if (s->cfg.precision != IFL_PREC_FP64) {
         // the three transfers together (each extrapolation reads and writes
         // only its own grid, and no transfer reads a grid), then the
@@ -1362,22 +1372,24 @@ int k_particles_to_grid(Solver* s) {
         IFL_TRY(k_extrapolate_flip(s, IFL_Q_T));
         IFL_TRY(k_extrapolate_flip(s, IFL_Q_U));
         IFL_TRY(k_extrapolate_flip(s, IFL_Q_V));
-        IFL_TRY(k_count_particles(s));
-        IFL_TRY(k_prune(s));
-        IFL_TRY(k_seed(s));
-        return IFL_OK;
+    } else {
+        IFL_TRY(k_from_particles(s, IFL_Q_D, true));
+        IFL_TRY(k_extrapolate_flip(s, IFL_Q_D));
+        IFL_TRY(k_extrapolate_flip(s, IFL_Q_T));
+        IFL_TRY(k_from_particles(s, IFL_Q_U, false));
+        IFL_TRY(k_extrapolate_flip(s, IFL_Q_U));
+        IFL_TRY(k_from_particles(s, IFL_Q_V, false));
+        IFL_TRY(k_extrapolate_flip(s, IFL_Q_V));
     }
-    IFL_TRY(k_from_particles(s, IFL_Q_D, true));
-    IFL_TRY(k_extrapolate_flip(s, IFL_Q_D));
-    IFL_TRY(k_extrapolate_flip(s, IFL_Q_T));
-    IFL_TRY(k_from_particles(s, IFL_Q_U, false));
-    IFL_TRY(k_extrapolate_flip(s, IFL_Q_U));
-    IFL_TRY(k_from_particles(s, IFL_Q_V, false));
-    IFL_TRY(k_extrapolate_flip(s, IFL_Q_V));
     IFL_TRY(k_count_particles(s));
-    IFL_TRY(k_prune(s));
-    IFL_TRY(k_seed(s));
-    return IFL_OK;
+    return fp32_particles(s) ? prune<float>(s) : prune<double>(s);
+}
+
+// particlesToGrid 8:916-927
+int k_particles_to_grid(Solver* s) {
+    IFL_TRY(k_particles_to_grid_begin(s));
+    IFL_TRY(k_prune_finish(s));
+    return k_seed(s);
 }
 
 template <typename P>
