@@ -36,6 +36,14 @@ KERNEL_BYTES_PER_CELL = {
 # the same for solver 4 (the batched form): its backward sweep also reads
 # the reference's _z at non-FLUID cells (A s, kept in q) for the unmasked dot
 BATCH_BYTES_PER_CELL = dict(KERNEL_BYTES_PER_CELL, sweep_bwd=56.0)
+# levels 6-8 in the FP64 form: the iterations after the first form the vector
+# update inside the product (adiag, aplusx, aplusy, old s, z, p read; p, new
+# s, q written; the FLUID flags): timed as the product
+FUSED_MATVEC_BYTES_PER_CELL = 73.0
+
+
+def kernel_bytes_per_cell(name, form):
+    return FUSED_MATVEC_BYTES_PER_CELL if name == "matvec" and form == "fp64" else KERNEL_BYTES_PER_CELL[name]
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "traffic.json")
 # The order-preserving floor of a MIC(0) sweep: cell (x, y) needs (x-1, y) and
 # (x, y-1), so any schedule that keeps each cell's operations takes W + H - 1
@@ -219,7 +227,7 @@ def run_form(form, n, steps, warmup):
         if p["launches"] == 0:
             continue
         avg_ms = p["ms"] / p["launches"]
-        bytes_launch = KERNEL_BYTES_PER_CELL[name] * n * n
+        bytes_launch = kernel_bytes_per_cell(name, form) * n * n
         kern[name] = {"launches": p["launches"], "total_ms": p["ms"], "avg_us": 1e3 * avg_ms,
                       "bytes_per_launch": bytes_launch, "achieved_GBps": bytes_launch / (avg_ms * 1e-3) / 1e9}
     solver.close()
@@ -363,7 +371,7 @@ def run_flip(args):
                       ("matvec", F.KERNEL_MATVEC)):
         p = solver.profile_read(kid)
         if p["launches"]:
-            b = KERNEL_BYTES_PER_CELL[name] * n * n
+            b = kernel_bytes_per_cell(name, form) * n * n
             avg = p["ms"] / p["launches"]
             kern[name] = {"launches": p["launches"], "total_ms": p["ms"], "avg_us": 1e3 * avg,
                           "bytes_per_launch": b, "achieved_GBps": b / (avg * 1e-3) / 1e9}

@@ -4,6 +4,7 @@
 #include <atomic>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <mutex>
 #include <new>
@@ -255,7 +256,7 @@ void free_all(Solver* s) {
         F(q.src); F(q.dst); F(q.phi); F(q.volume); F(q.nx); F(q.ny); F(q.cell); F(q.body); F(q.mask);
     }
     F(s->udens); F(s->vdens);
-    F(s->r); F(s->p); F(s->z); F(s->s); F(s->precon); F(s->adiag); F(s->aplusx); F(s->aplusy);
+    F(s->r); F(s->p); F(s->z); F(s->s); F(s->s2); F(s->precon); F(s->adiag); F(s->aplusx); F(s->aplusy);
     F(s->cx); F(s->cy); F(s->yv); F(s->qv); F(s->dflag); F(s->fmask); F(s->edge);
     F(s->cx32); F(s->cy32); F(s->pc32); F(s->y32); F(s->partial); F(s->st);
     F(s->e32); F(s->r32); F(s->s32); F(s->z32); F(s->q32); F(s->ad32); F(s->apx32); F(s->apy32); F(s->st_outer);
@@ -333,6 +334,11 @@ int init_solver(Solver* s) {
                        &s->cx, &s->cy, &s->yv, &s->qv};
     for (double** a : arrs) IFL_TRY(dalloc(s, a, n));
     IFL_TRY(balloc(s, &s->dflag, n));
+    // the fused vector update of the PCG iterations (levels 6-8, FP64);
+    // IFL_NO_FUSED_UPDATE=1 in the environment at creation keeps the
+    // separate update kernel (diagnostics: the two give identical iterates)
+    const char* nofuse = getenv("IFL_NO_FUSED_UPDATE");
+    if (lv >= 6 && s->cfg.precision == IFL_PREC_FP64 && !(nofuse && nofuse[0] == '1')) IFL_TRY(dalloc(s, &s->s2, n));
     if (s->cfg.precision != IFL_PREC_FP64) {
         float** sarrs[] = {&s->cx32, &s->cy32, &s->pc32, &s->y32, &s->e32, &s->r32, &s->s32,
                            &s->z32, &s->q32, &s->ad32, &s->apx32, &s->apy32};

@@ -58,6 +58,10 @@ struct Solver {
     Skew sk;
     size_t sk_count = 0;
     double *r = nullptr, *p = nullptr, *z = nullptr, *s = nullptr;
+    // levels 6-8, FP64 with tree-ordered dots: the PCG iterations after the
+    // first form s = z + beta s inside the matrix-vector product, writing the
+    // new s here and in s by turns (null: not used)
+    double* s2 = nullptr;
     double *precon = nullptr, *adiag = nullptr, *aplusx = nullptr, *aplusy = nullptr;
     double *cx = nullptr, *cy = nullptr;  // aPlusX*precon, aPlusY*precon (the sweeps' products)
     double *yv = nullptr;                 // forward-sweep output

@@ -145,6 +145,55 @@ __global__ __launch_bounds__(256) void matvec_kernel(VecArgs a, int check_done) 
     matvec_body<OUT_Q>(a, blockIdx.x, check_done);
 }
 
+// A1 and A2 in one pass, for vector operations masked to FLUID cells (levels
+// 6-8) in the FP64 form: scaledAdd(_p, _p, _s, alpha) 7:869 and
+// scaledAdd(_s, _z, _s, beta) 7:881 of the previous iteration, then
+// matrixVectorProduct(_z, _s) 7:868 and the block partial of its dot
+// product. Each product term takes its neighbour's new s as that update
+// forms it (z + s beta at a FLUID cell, s unchanged elsewhere), so the terms
+// are vec_update_kernel's and matvec_kernel's values; the new s goes to a
+// second array, since neighbouring workgroups still read the old one.
+__global__ __launch_bounds__(256) void matvec_update_kernel(VecArgs a, const double* __restrict__ so,
+                                                            double* __restrict__ sn) {
+    __shared__ double red[4];
+    PcgState* st = a.st;
+    if (st->done) return;
+    const Skew& sk = a.sk;
+    const long n = (long)sk.nstrips * sk.strip_sz;
+    const long blk = blockIdx.x;
+    const long i = blk * (long)blockDim.x + threadIdx.x;
+    const double alpha = st->alpha, beta = st->beta;
+    auto snew = [&](long j) -> double { return a.flag[j] ? a.z[j] + so[j] * beta : so[j]; };
+    double prod = 0.0;
+    int x, y;
+    if (i < n && skew_cell(sk, i, x, y)) {
+        const bool fl = a.flag[i] != 0;
+        const double s0 = so[i];
+        if (fl) a.p[i] = a.p[i] + s0 * alpha;
+        const double b = fl ? a.z[i] + s0 * beta : s0;
+        sn[i] = b;
+        const int l = y & 63;
+        const int W = sk.W, H = sk.H;
+        double t = a.adiag[i] * b;
+        if (x > 0) t += a.apx[i - kStrip] * snew(i - kStrip);
+        if (y > 0) {
+            long js = l > 0 ? i - kStrip - 1 : sk.pos(x, y - 1);
+            t += a.apy[js] * snew(js);
+        }
+        if (x < W - 1) t += a.apx[i] * snew(i + kStrip);
+        if (y < H - 1) {
+            long jn = l < kStrip - 1 ? i + kStrip + 1 : sk.pos(x, y + 1);
+            t += a.apy[i] * snew(jn);
+        }
+        a.q[i] = t;
+        if (fl) prod = t * b;
+    }
+    prod = wave_sum(prod);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = prod;
+    __syncthreads();
+    if (threadIdx.x == 0) a.partial[blk] = (red[0] + red[1]) + (red[2] + red[3]);
+}
+
 // Fixed-order sum of `n` partials by one workgroup of 256 threads.
 IFL_DEV double block_sum_partials(const double* partial, int n) {
     __shared__ double red[256];
@@ -761,17 +810,29 @@ __global__ void defl_init_s_kernel(VecArgs a) {
 
 // Iteration `it` of the solve; each kernel returns at once when the solve is
 // already done.
+// iterations fuse the vector update into the product (matvec_update_kernel)
+static bool fused_update(const Solver* s, const VecArgs& a) {
+    return s->s2 && s->cfg.dot_order == IFL_DOT_TREE && a.vm && !a.defl;
+}
+// the array holding s after iteration `it` (s and s2 by turns when fused)
+static double* s_after(Solver* s, int it) { return s->s2 && it > 0 && (it & 1) ? s->s2 : s->s; }
+
 static int launch_iteration(Solver* s, int it, int limit) {
     VecArgs a = vec_args(s);
     long n = skew_n(s);
     int nbv = grid_blocks(n, 256);
     int seq = s->cfg.dot_order == IFL_DOT_SEQUENTIAL;
-    if (it > 0) {
+    const bool fuse = fused_update(s, a);
+    if (it > 0 && !fuse) {
         vec_update_kernel<<<nbv, 256, 0, s->stream>>>(a, 0, 1);
         IFL_LAUNCH_CHECK();
     }
     IFL_TRY(prof_begin(s, IFL_KERNEL_MATVEC, it));
-    matvec_kernel<1><<<nbv, 256, 0, s->stream>>>(a, 1);
+    if (it > 0 && fuse) {
+        matvec_update_kernel<<<nbv, 256, 0, s->stream>>>(a, s_after(s, it - 1), s_after(s, it));
+    } else {
+        matvec_kernel<1><<<nbv, 256, 0, s->stream>>>(a, 1);
+    }
     IFL_LAUNCH_CHECK();
     IFL_TRY(prof_end(s));
     fin_alpha_kernel<<<1, 256, 0, s->stream>>>(a, nbv, seq);
@@ -1088,6 +1149,14 @@ int k_project(Solver* s, int limit, ifl_solve_stats* stats) {
         (void)edge_reset(s);
         return IFL_ERR_DEVICE;
     }
+    // with the fused update, s is where the last product that ran read it;
+    // the tails below work on that array, and it ends in s
+    double* s_last = nullptr;
+    if (fused_update(s, a)) {
+        const int worked = h.status == IFL_SOLVE_CONVERGED ? h.printed_iter + 1 : h.iter;
+        s_last = s_after(s, worked - 1);
+        a.s = s_last;
+    }
     // deferred tails of the last iteration
     if (h.status == IFL_SOLVE_CONVERGED) {
         vec_update_kernel<<<grid_blocks(n, 256), 256, 0, s->stream>>>(a, 2, 0);  // p += alpha*s (7:869)
@@ -1101,6 +1170,8 @@ int k_project(Solver* s, int limit, ifl_solve_stats* stats) {
             IFL_LAUNCH_CHECK();
         }
     }
+    if (s_last && s_last != s->s)
+        IFL_HIP(hipMemcpyAsync(s->s, s_last, n * sizeof(double), hipMemcpyDeviceToDevice, s->stream));
     if (a.defl) {
         // r back to the residual itself
         defl_shift_kernel<<<grid_blocks(n, 256), 256, 0, s->stream>>>(a, 1.0);
