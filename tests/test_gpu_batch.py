@@ -222,3 +222,31 @@ def test_fused_batch_full_draw():
     its = [r[1] for row in rows for r in row]
     print(f"256 instances, {sum(i['w'] * i['h'] for i in insts)} cells: pressure iterations {min(its)}..{max(its)}",
           flush=True)
+
+
+def test_batch_add_inflow_arguments():
+    """ifl_batch_add_inflow rejects a solver index outside the batch (and a
+    missing array) with IFL_ERR_ARG before touching any solver; an empty
+    call is a no-op."""
+    import ctypes
+    import numpy as np
+    import ifl_fluids as F
+    solvers = [F.FluidSolver(4, 64, 48, (0.1,), []) for _ in range(2)]
+    batch = F.Batch(solvers)
+    before = [s.field(ol.qfield(ol.Q_D, ol.QF_SRC)) for s in solvers]
+    for bad in ([(2, 0.4, 0.2, 0.1, 0.05, 1.0, 0.0, 0.0, 0.0)], [(-1, 0.4, 0.2, 0.1, 0.05, 1.0, 0.0, 0.0, 0.0)]):
+        with pytest.raises(F.IflError):
+            batch.add_inflow(bad)
+    rect = (ctypes.c_double * 4)(0.4, 0.2, 0.1, 0.05)
+    inst = (ctypes.c_int * 1)(0)
+    assert F.lib.ifl_batch_add_inflow(batch._h, 1, inst, rect, None) == 1  # IFL_ERR_ARG
+    assert F.lib.ifl_batch_add_inflow(batch._h, 0, None, None, None) == 0
+    batch.add_inflow([])
+    for s, b in zip(solvers, before):
+        assert np.array_equal(s.field(ol.qfield(ol.Q_D, ol.QF_SRC)), b)
+    batch.add_inflow([(1, 0.4, 0.2, 0.1, 0.05, 1.0, 0.0, 0.0, 0.0)])
+    assert np.array_equal(solvers[0].field(ol.qfield(ol.Q_D, ol.QF_SRC)), before[0])
+    assert solvers[1].field(ol.qfield(ol.Q_D, ol.QF_SRC)).max() > 0
+    batch.close()
+    for s in solvers:
+        s.close()
