@@ -78,12 +78,15 @@ IFL_DEV void vec_update_body(const VecArgs& a, long blk, int mode, int check_don
     if (i >= n) return;
     int x, y;
     if (!skew_cell(a.sk, i, x, y)) return;
+    // every load first, then the selects
     const bool fl = a.flag[i] != 0;
+    const double s = a.s[i];
+    const double p = mode != 1 ? a.p[i] : 0.0;
+    const double zv = mode != 2 ? a.z[i] : 0.0, qv = mode != 2 && a.pm ? a.q[i] : 0.0;
     if (a.vm && !fl) return;
-    double s = a.s[i];
-    if (mode != 1) a.p[i] = a.p[i] + s * st->alpha;
+    if (mode != 1) a.p[i] = p + s * st->alpha;
     if (mode != 2) {
-        double zi = a.pm && !fl ? a.q[i] : a.z[i];
+        double zi = a.pm && !fl ? qv : zv;
         if (a.defl && fl) zi = zi - st->zmean;
         a.s[i] = zi + s * st->beta;
     }
@@ -118,22 +121,25 @@ IFL_DEV void matvec_body(const VecArgs& a, long blk, int check_done) {
         const int l = y & 63;
         const int W = sk.W, H = sk.H;
         const double* b = a.s;
-        double t = a.adiag[i] * b[i];
-        if (x > 0) t += a.apx[i - kStrip] * b[i - kStrip];
-        if (y > 0) {
-            long js = l > 0 ? i - kStrip - 1 : sk.pos(x, y - 1);
-            t += a.apy[js] * b[js];
-        }
-        if (x < W - 1) t += a.apx[i] * b[i + kStrip];
-        if (y < H - 1) {
-            long jn = l < kStrip - 1 ? i + kStrip + 1 : sk.pos(x, y + 1);
-            t += a.apy[i] * b[jn];
-        }
+        // every load first (a missing neighbour reads cell i and is not added)
+        const bool hw = x > 0, hs = y > 0, he = x < W - 1, hn = y < H - 1;
+        const long jw = hw ? i - kStrip : i;
+        const long js = hs ? (l > 0 ? i - kStrip - 1 : sk.pos(x, y - 1)) : i;
+        const long je = he ? i + kStrip : i;
+        const long jn = hn ? (l < kStrip - 1 ? i + kStrip + 1 : sk.pos(x, y + 1)) : i;
+        const double b0 = b[i], bw = b[jw], bs = b[js], be = b[je], bn = b[jn];
+        const double ad = a.adiag[i], axw = a.apx[jw], ays = a.apy[js], ax = a.apx[i], ay = a.apy[i];
+        const bool on = !a.vm || a.flag[i];
+        double t = ad * b0;
+        t = hw ? t + axw * bw : t;
+        t = hs ? t + ays * bs : t;
+        t = he ? t + ax * be : t;
+        t = hn ? t + ay * bn : t;
         if (OUT_Q)
             a.q[i] = t;
         else
             a.z[i] = t;
-        if (!a.vm || a.flag[i]) prod = t * b[i];
+        if (on) prod = t * b0;
     }
     prod = wave_sum(prod);
     if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = prod;
@@ -163,28 +169,37 @@ __global__ __launch_bounds__(256) void matvec_update_kernel(VecArgs a, const dou
     const long blk = blockIdx.x;
     const long i = blk * (long)blockDim.x + threadIdx.x;
     const double alpha = st->alpha, beta = st->beta;
-    auto snew = [&](long j) -> double { return a.flag[j] ? a.z[j] + so[j] * beta : so[j]; };
     double prod = 0.0;
     int x, y;
     if (i < n && skew_cell(sk, i, x, y)) {
-        const bool fl = a.flag[i] != 0;
-        const double s0 = so[i];
-        if (fl) a.p[i] = a.p[i] + s0 * alpha;
-        const double b = fl ? a.z[i] + s0 * beta : s0;
-        sn[i] = b;
         const int l = y & 63;
         const int W = sk.W, H = sk.H;
-        double t = a.adiag[i] * b;
-        if (x > 0) t += a.apx[i - kStrip] * snew(i - kStrip);
-        if (y > 0) {
-            long js = l > 0 ? i - kStrip - 1 : sk.pos(x, y - 1);
-            t += a.apy[js] * snew(js);
-        }
-        if (x < W - 1) t += a.apx[i] * snew(i + kStrip);
-        if (y < H - 1) {
-            long jn = l < kStrip - 1 ? i + kStrip + 1 : sk.pos(x, y + 1);
-            t += a.apy[i] * snew(jn);
-        }
+        // every load first (a missing neighbour reads cell i and is not
+        // added), so the loads overlap instead of waiting on the flags
+        const bool hw = x > 0, hs = y > 0, he = x < W - 1, hn = y < H - 1;
+        const long jw = hw ? i - kStrip : i;
+        const long js = hs ? (l > 0 ? i - kStrip - 1 : sk.pos(x, y - 1)) : i;
+        const long je = he ? i + kStrip : i;
+        const long jn = hn ? (l < kStrip - 1 ? i + kStrip + 1 : sk.pos(x, y + 1)) : i;
+        const bool fl = a.flag[i] != 0, fw = a.flag[jw] != 0, fs = a.flag[js] != 0, fe = a.flag[je] != 0,
+                   fn = a.flag[jn] != 0;
+        const double s0 = so[i], sw = so[jw], ss = so[js], se = so[je], snn = so[jn];
+        const double z0 = a.z[i], zw = a.z[jw], zs = a.z[js], ze = a.z[je], zn = a.z[jn];
+        const double ad = a.adiag[i], axw = a.apx[jw], ays = a.apy[js], ax = a.apx[i], ay = a.apy[i];
+        const double p0 = a.p[i];
+        // the new s (z + s beta at a FLUID cell, s elsewhere)
+        const double b = fl ? z0 + s0 * beta : s0;
+        const double bw = fw ? zw + sw * beta : sw;
+        const double bs = fs ? zs + ss * beta : ss;
+        const double be = fe ? ze + se * beta : se;
+        const double bn = fn ? zn + snn * beta : snn;
+        if (fl) a.p[i] = p0 + s0 * alpha;
+        sn[i] = b;
+        double t = ad * b;
+        t = hw ? t + axw * bw : t;
+        t = hs ? t + ays * bs : t;
+        t = he ? t + ax * be : t;
+        t = hn ? t + ay * bn : t;
         a.q[i] = t;
         if (fl) prod = t * b;
     }
