@@ -212,10 +212,18 @@ __global__ __launch_bounds__(256) void matvec_update_kernel(VecArgs a, const dou
 // Fixed-order sum of `n` partials by one workgroup of 256 threads.
 IFL_DEV double block_sum_partials(const double* partial, int n) {
     __shared__ double red[256];
-    // thread t sums partials t, t + 256, ... in that order; eight loads are
-    // in flight at a time (the adds keep their order)
+    // thread t sums partials t, t + 256, ... in that order; 32 loads are in
+    // flight at a time (the adds keep their order)
+    constexpr int kInFlight = 32;
     double acc = 0.0;
     int k = threadIdx.x;
+    for (; k + (kInFlight - 1) * 256 < n; k += kInFlight * 256) {
+        double v[kInFlight];
+#pragma unroll
+        for (int u = 0; u < kInFlight; u++) v[u] = partial[k + u * 256];
+#pragma unroll
+        for (int u = 0; u < kInFlight; u++) acc += v[u];
+    }
     for (; k + 7 * 256 < n; k += 8 * 256) {
         double v[8];
 #pragma unroll
