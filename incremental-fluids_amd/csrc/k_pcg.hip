@@ -332,9 +332,20 @@ __global__ void fin_check_kernel(PcgState* st, double tol) { fin_check_body(st, 
 // finish C: sigmaNew = dot(z, r), beta = sigmaNew/sigma (7:880-882).
 // init=1 is the entry of project(): sigma = dot(z, r) and the early-exit
 // test of 7:858-862.
+// The convergence test of 7:872-876 where the PCG loop has no fin_check
+// launch: the backward sweep returns at once when it holds (as it does once
+// fin_check has set `done`), and the finish after it records the exit here;
+// `iter` has not moved since the test's place in the loop.
+IFL_DEV bool fin_converged(PcgState* st, double tol) {
+    if (!(st->max_err < tol)) return false;
+    if (threadIdx.x == 0) fin_check_body(st, tol);
+    return true;
+}
+
 IFL_DEV void fin_sigma_body(const VecArgs& a, int n, int seq, int init, double tol, int limit) {
     PcgState* st = a.st;
     if (!init && st->done) return;
+    if (!init && fin_converged(st, tol)) return;
     double d;
     if (seq) {
         // inside the loop the reference's _z at a non-FLUID cell is A s
@@ -535,6 +546,7 @@ static StreamArgs bwd_args(Solver* s) {
     a.in[4] = s->r;
     a.in[5] = MODE == 1 ? s->qv : s->z;  // the reference's _z at non-FLUID cells (levels 4-5 dot)
     a.out[0] = s->z;
+    if (MODE == 1) a.sigma = s->cfg.tol;  // the PCG loop's convergence test (sweep_body)
     return a;
 }
 template <int MODE, typename T>
@@ -720,6 +732,7 @@ __global__ __launch_bounds__(256) void fin_sigma32_kernel(VecArgs a, int n, int 
                                                           int deflate, const PcgState* outer) {
     PcgState* st = a.st;
     if (!init && st->done) return;
+    if (!init && fin_converged(st, tol)) return;
     if (deflate) {
         const double zs = block_sum_partials(a.partial + n, n);
         if (threadIdx.x == 0) st->zmean = zs / outer->dot_tmp;
@@ -863,8 +876,7 @@ static int launch_iteration(Solver* s, int it, int limit) {
     IFL_TRY(prof_begin(s, IFL_KERNEL_SWEEP_FWD, it));
     IFL_TRY(launch_fwd<1>(s));
     IFL_TRY(prof_end(s));
-    fin_check_kernel<<<1, 1, 0, s->stream>>>(s->st, s->cfg.tol);
-    IFL_LAUNCH_CHECK();
+    // the convergence test (7:872-876) is in the backward sweep and the finish after it
     IFL_TRY(prof_begin(s, IFL_KERNEL_SWEEP_BWD, it));
     IFL_TRY(launch_bwd<1>(s));
     IFL_TRY(prof_end(s));
@@ -1572,8 +1584,7 @@ int batch_iteration(Solver* s, BatchDev* d, hipStream_t stream, int it, bool pro
     batch_sweep_kernel<SW_FWD, 1, LV, NGF><<<d->nsblk, 64 * kWaves, d->shmem_f, stream>>>(d->fwd, d->fmap);
     IFL_LAUNCH_CHECK();
     if (prof) IFL_TRY(batch_time(s, d, stream, IFL_KERNEL_SWEEP_FWD, false));
-    batch_fin_check_kernel<<<grid_blocks(d->n, 256), 256, 0, stream>>>(d->inst, d->n);
-    IFL_LAUNCH_CHECK();
+    // the convergence test is in the backward sweep and the finish after it
     if (prof) IFL_TRY(batch_time(s, d, stream, IFL_KERNEL_SWEEP_BWD, true));
     batch_sweep_kernel<SW_BWD, 1, LV, NGB><<<d->nsblk, 64 * kWaves, d->shmem_b, stream>>>(d->bwd, d->bmap);
     IFL_LAUNCH_CHECK();

@@ -103,7 +103,8 @@ struct StreamArgs {
     double* partial;
     PcgState* st;
     int* timeout;
-    double tau, sigma;
+    double tau, sigma;      // the factor build's tau and sigma (7:743-744); the PCG
+                            // loop's backward sweep: sigma = the convergence tolerance
     int strip0, nlaunch;    // the strips of this launch
     int pad;                // unused: keeps the fields below where the sweeps were
                             // measured fastest (without it 388 / 363 us per forward /
@@ -724,6 +725,9 @@ IFL_DEV void sweep_body(const StreamArgs& a, int j) {
     extern __shared__ __align__(16) char smem[];
     PcgState* st = a.st;
     if (KIND != SW_PRECON && ((MODE >= 1 && st->done) || st->timeout)) return;
+    // inside the PCG loop the backward sweep follows the convergence test
+    // (7:872-876): it does nothing once max|r| < tol (a.sigma holds tol)
+    if (KIND == SW_BWD && MODE == 1 && st->max_err < a.sigma) return;
     RingHdr* hdr = (RingHdr*)smem;
     double* ering = (double*)(smem + sizeof(RingHdr));
     char* ring = smem + sizeof(RingHdr) + kERing * sizeof(double);
