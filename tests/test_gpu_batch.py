@@ -250,3 +250,81 @@ def test_batch_add_inflow_arguments():
     batch.close()
     for s in solvers:
         s.close()
+
+
+@pytest.mark.parametrize("level", [4, 6])
+def test_fused_batch_host_bodies(level):
+    """Solvers holding a body evaluated on the host (ifl_set_host_body_ops;
+    here a sphere written in numpy) run their stages per solver inside the
+    batch: three instances of different sizes, stepped as one batch, end
+    exactly where the same solvers stepped one after the other end."""
+    import ctypes
+    import numpy as np
+    import ifl_fluids as F
+    dp = ctypes.POINTER(ctypes.c_double)
+    DIST = ctypes.CFUNCTYPE(None, ctypes.c_void_p, ctypes.c_int, dp, dp, dp, ctypes.c_size_t)
+    CLOSEST = ctypes.CFUNCTYPE(None, ctypes.c_void_p, ctypes.c_int, dp, dp, ctypes.c_size_t)
+    NORMAL = ctypes.CFUNCTYPE(None, ctypes.c_void_p, ctypes.c_int, dp, dp, dp, dp, ctypes.c_size_t)
+
+    class Ops(ctypes.Structure):
+        _fields_ = [("user", ctypes.c_void_p), ("distance", DIST), ("closest", CLOSEST), ("normal", NORMAL)]
+
+    cx, cy, r = 0.42, 0.5, 0.12
+
+    def view(p, n):
+        return np.ctypeslib.as_array(p, shape=(n,))
+
+    def dist(user, body, x, y, out, n):
+        if n:
+            view(out, n)[:] = np.hypot(view(x, n) - cx, view(y, n) - cy) - r
+
+    def closest(user, body, x, y, n):
+        if n:
+            X, Y = view(x, n), view(y, n)
+            d = np.maximum(np.hypot(X - cx, Y - cy), 1e-300)
+            X[:], Y[:] = cx + (X - cx) * (r / d), cy + (Y - cy) * (r / d)
+
+    def normal(user, body, x, y, nx, ny, n):
+        if n:
+            X, Y = view(x, n), view(y, n)
+            d = np.maximum(np.hypot(X - cx, Y - cy), 1e-300)
+            view(nx, n)[:], view(ny, n)[:] = (X - cx) / d, (Y - cy) / d
+
+    ops = Ops(None, DIST(dist), CLOSEST(closest), NORMAL(normal))
+
+    class HostSphere(F.SolidBody):
+        type = 2  # IFL_BODY_HOST
+
+    params = (0.1,) if level <= 5 else (0.1, 1.0, 0.01)
+    inflow = (0.3, 0.2, 0.1, 0.05, 1.0, 600.0, 0.0, 0.5)
+    sizes = [(64, 48), (80, 64), (48, 72)]
+
+    def make(w, h):
+        s = F.FluidSolver(level, w, h, params, [])
+        assert F.lib.ifl_set_host_body_ops(s._h, ctypes.byref(ops)) == 0
+        s.bodies = [HostSphere(cx, cy, r, r, 0.0, 0.0, 0.0, 0.0)]
+        return s
+    steps, dt = 3, 0.005
+    batched = [make(w, h) for w, h in sizes]
+    batch = F.Batch(batched)
+    rows_b = []
+    for _ in range(steps):
+        for s in batched:
+            s.add_inflow(*inflow)
+        st = batch.update(dt)
+        rows_b.append([(x.heat.iterations, x.pressure.iterations, x.pressure.status) for x in st])
+    batch.close()
+    fields = ol.state_fields(level)
+    for k, (w, h) in enumerate(sizes):
+        single = make(w, h)
+        rows = []
+        for _ in range(steps):
+            single.add_inflow(*inflow)
+            st = single.update(dt)
+            rows.append((st.heat.iterations, st.pressure.iterations, st.pressure.status))
+        assert rows == [row[k] for row in rows_b]
+        bad = compare_fields(single, batched[k], fields, where=f"instance {k} {w}x{h}")
+        assert not bad, "\n".join(bad[:6])
+        single.close()
+        batched[k].close()
+    print(f"level {level}: {rows_b}", flush=True)
