@@ -72,6 +72,10 @@ def parse():
                     help="batch: solver 4 with the drawn sizes (config 4), or solver 8 (FLIP) at 256^2 (config 5)")
     ap.add_argument("--batch-mode", choices=["fused", "streams"], default="fused",
                     help="batch: one ifl_batch (PCG iterations fused over the instances) or one stream per instance")
+    ap.add_argument("--flip-alpha", type=float, default=None,
+                    help="flip: the FLIP blend (the reference's 0.001; 0.05 for the '0.95 FLIP' variant)")
+    ap.add_argument("--g2p", choices=["bilinear", "cubic"], default="bilinear",
+                    help="flip: gridToParticles interpolation (cubic: the Catmull-Rom variant, not the reference)")
     ap.add_argument("--form", choices=["fp64", "mixed", "refine"], default=None,
                     help="time only this form (default: FP64 as the value, the mixed form beside it)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -348,7 +352,12 @@ def run_flip(args):
     objs = [F.SolidBox(*b[1:]) for b in scene["bodies"]]
     form = args.form or "fp64"
     prec = {"fp64": F.PREC_FP64, "mixed": F.PREC_MIXED, "refine": F.PREC_REFINE}[form]
-    solver = F.FluidSolver(8, n, n, scene["params"], objs, precision=prec)
+    variant = {}
+    if args.flip_alpha is not None:
+        variant["flip_alpha"] = args.flip_alpha
+    if args.g2p == "cubic":
+        variant["flip_g2p"] = 1
+    solver = F.FluidSolver(8, n, n, scene["params"], objs, precision=prec, **variant)
     for _ in range(args.warmup):
         solver.update(scene["dt"])
     solver.synchronize()
@@ -387,7 +396,8 @@ def run_flip(args):
     dom = max(("sweep_fwd", "sweep_bwd", "matvec"), key=lambda k: kern.get(k, {"total_ms": -1})["total_ms"])
     k = kern[dom]
     out = {
-        "metric": f"solver 8 (FLIP) steps/s at 1024^2, {'FP64 drop-in' if form == 'fp64' else form + ' form'}",
+        "metric": f"solver 8 (FLIP) steps/s at 1024^2, {'FP64 drop-in' if form == 'fp64' else form + ' form'}"
+                  + ("" if not variant else f", variant {variant} (not the reference's)"),
         "value": args.steps / wall, "unit": "steps/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
         "ms_per_step": 1e3 * wall / args.steps, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "f64" if form == "fp64" else "f32+f64", "data": "synthetic",

@@ -95,7 +95,13 @@ typedef struct ifl_config {
     double t_amb;         /* 294 (7:950) */
     double gravity;       /* 9.81 (7:951) */
     double flip_alpha;    /* 0.001 (8:1296) */
-    int32_t reserved[10];
+    int32_t flip_g2p;     /* level 8, the FLIP update's gridToParticles (8:1401):
+                             0 bilinear lerp, as the reference (8:904-911);
+                             1 the Catmull-Rom cerp of solvers 3-7 (7:404-422),
+                             a variant with no counterpart in the reference (the
+                             constructor's gridToParticles(1.0), 8:1314, stays
+                             bilinear) */
+    int32_t reserved[9];
 } ifl_config;
 
 /* Fills cfg with the reference's literal defaults for `level`. */

@@ -513,7 +513,7 @@ int update_part(Solver* s, int after, double dt, int* next) {
         // 8:1393-1412: FLIP update of the particles, then their advection
         const double alpha = s->cfg.flip_alpha;
         for (int qi = 0; qi < 4; qi++) IFL_TRY(k_diff(s, qi, alpha, 0));
-        IFL_TRY(k_grid_to_particles(s, alpha));
+        IFL_TRY(k_grid_to_particles(s, alpha, s->cfg.flip_g2p == 1));
         for (int qi = 0; qi < 4; qi++) IFL_TRY(k_diff(s, qi, alpha, 1));
         IFL_TRY(k_particle_advect(s, dt));
         if (!s->flip_split) IFL_TRY(flip_check_overflow(s));  // else at the batch's step end
@@ -773,6 +773,10 @@ int ifl_create(const ifl_config* cfg, ifl_solver** out) {
     }
     if (cfg->precision != IFL_PREC_FP64 && cfg->precision != IFL_PREC_MIXED && cfg->precision != IFL_PREC_REFINE) {
         g_err = "ifl_create: precision must be IFL_PREC_FP64, IFL_PREC_MIXED or IFL_PREC_REFINE";
+        return IFL_ERR_ARG;
+    }
+    if (cfg->flip_g2p != 0 && cfg->flip_g2p != 1) {
+        g_err = "ifl_create: flip_g2p must be 0 (bilinear, the reference) or 1 (Catmull-Rom)";
         return IFL_ERR_ARG;
     }
     int ndev = 0;

@@ -296,7 +296,7 @@ void flip_free(Solver* s);
 int flip_init_particles(Solver* s);
 int k_from_particles(Solver* s, int qi, bool with_t);
 int k_extrapolate_flip(Solver* s, int qi);
-int k_grid_to_particles(Solver* s, double alpha);
+int k_grid_to_particles(Solver* s, double alpha, bool cubic = false);  // cubic: cerp instead of lerp
 int k_particles_to_grid(Solver* s);
 // particlesToGrid up to the prune's kernels (no host wait), and the prune's
 // count read back: k_particles_to_grid_begin, k_prune_finish, k_seed together

@@ -129,6 +129,7 @@ struct G2PArgs {
     const P* posy;
     P* prop[4];
     int n;
+    int cubic;  // the variant's Catmull-Rom samples (qcerp) instead of lerp
     double alpha;
 };
 
@@ -142,7 +143,7 @@ __global__ void g2p_kernel(G2PArgs<P> a) {
         const QtyView& q = a.q[t];
         double p = a.prop[t][i];
         p *= 1.0 - a.alpha;
-        p += qlerp(q.src, q.w, q.h, q.ox, q.oy, x, y);
+        p += a.cubic ? qcerp(q.src, q.w, q.h, q.ox, q.oy, x, y) : qlerp(q.src, q.w, q.h, q.ox, q.oy, x, y);
         a.prop[t][i] = (P)p;
     }
 }
@@ -1117,7 +1118,7 @@ int flip_init_particles(Solver* s) {
 }
 
 template <typename P>
-static int grid_to_particles(Solver* s, double alpha) {
+static int grid_to_particles(Solver* s, double alpha, bool cubic) {
     int n = s->particle_count;
     if (n == 0) return IFL_OK;
     const ParticleArrays<P> pa = particle_arrays<P>(s);
@@ -1129,6 +1130,7 @@ static int grid_to_particles(Solver* s, double alpha) {
     a.posx = pa.a[0];
     a.posy = pa.a[1];
     a.n = n;
+    a.cubic = cubic ? 1 : 0;
     a.alpha = alpha;
     IFL_TRY(prof_begin(s, IFL_KERNEL_G2P, -1));
     g2p_kernel<<<grid_blocks(n, 256), 256, 0, s->stream>>>(a);
@@ -1137,8 +1139,8 @@ static int grid_to_particles(Solver* s, double alpha) {
     return IFL_OK;
 }
 
-int k_grid_to_particles(Solver* s, double alpha) {
-    return fp32_particles(s) ? grid_to_particles<float>(s, alpha) : grid_to_particles<double>(s, alpha);
+int k_grid_to_particles(Solver* s, double alpha, bool cubic) {
+    return fp32_particles(s) ? grid_to_particles<float>(s, alpha, cubic) : grid_to_particles<double>(s, alpha, cubic);
 }
 
 int k_copy_old(Solver* s, int qi) {

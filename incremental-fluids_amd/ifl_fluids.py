@@ -48,7 +48,7 @@ class Config(ctypes.Structure):
         ("density", ctypes.c_double), ("density_soot", ctypes.c_double), ("diffusion", ctypes.c_double),
         ("tau", ctypes.c_double), ("sigma", ctypes.c_double), ("tol", ctypes.c_double),
         ("t_amb", ctypes.c_double), ("gravity", ctypes.c_double), ("flip_alpha", ctypes.c_double),
-        ("reserved", ctypes.c_int32 * 10),
+        ("flip_g2p", ctypes.c_int32), ("reserved", ctypes.c_int32 * 9),
     ]
 
 

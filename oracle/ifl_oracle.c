@@ -201,6 +201,7 @@ typedef struct orc_solver {
     int level, w, h;
     double hx;
     double density, density_soot, diffusion, t_amb, g, flip_alpha;
+    int g2p_cubic; /* the cubic gridToParticles variant (orc_set_flip) */
     qty_t q[4];
     double *r, *p, *z, *s, *precon, *adiag, *aplusx, *aplusy, *udens, *vdens;
     int nbodies;
@@ -1032,11 +1033,14 @@ static void from_particles(orc_solver* s, qty_t* q, const double* property) {
 }
 
 /* gridToParticles 8:904-911 */
-static void grid_to_particles(orc_solver* s, double alpha) {
+/* cubic: the variant that samples with the Catmull-Rom cerp of solvers 3-7
+ * (7:404-422) instead of lerp; not in the reference */
+static void grid_to_particles(orc_solver* s, double alpha, int cubic) {
     for (int t = 0; t < 4; t++)
         for (int i = 0; i < s->particle_count; i++) {
             s->prop[t][i] *= 1.0 - alpha;
-            s->prop[t][i] += qty_lerp(&s->q[t], s->posx[i], s->posy[i]);
+            s->prop[t][i] += cubic ? qty_cerp(&s->q[t], s->posx[i], s->posy[i])
+                                   : qty_lerp(&s->q[t], s->posx[i], s->posy[i]);
         }
 }
 
@@ -1231,9 +1235,16 @@ orc_solver* orc_create(int level, int w, int h, const double* params, const orc_
         s->weight = dalloc((size_t)(w + 1) * (h + 1));
         s->counts = (int*)calloc(n, sizeof(int));
         init_particles(s);
-        grid_to_particles(s, 1.0);
+        grid_to_particles(s, 1.0, 0);
     }
     return s;
+}
+
+/* level 8: the FLIP update's blend factor and gridToParticles interpolation
+ * (0 lerp as the reference, 1 the Catmull-Rom variant) */
+void orc_set_flip(orc_solver* s, double alpha, int cubic) {
+    s->flip_alpha = alpha;
+    s->g2p_cubic = cubic;
 }
 
 void orc_destroy(orc_solver* s) {
@@ -1316,7 +1327,7 @@ void orc_update(orc_solver* s, double dt, int* iters) {
     set_bc(s);
     if (lv == 8) {
         for (int q = 0; q < 4; q++) qty_diff(&s->q[q], s->flip_alpha);
-        grid_to_particles(s, s->flip_alpha);
+        grid_to_particles(s, s->flip_alpha, s->g2p_cubic);
         for (int q = 0; q < 4; q++) qty_undiff(&s->q[q], s->flip_alpha);
         particle_advect(s, dt);
     } else {
@@ -1428,7 +1439,7 @@ int orc_stage(orc_solver* s, int stage, int iarg, double darg, double* result) {
     if (lv != 8) return 4;
     switch (stage) {
     case IFL_STAGE_FROM_PARTICLES: if (!q) return 1; from_particles(s, q, s->prop[iarg]); return 0;
-    case IFL_STAGE_G2P: grid_to_particles(s, darg); return 0;
+    case IFL_STAGE_G2P: grid_to_particles(s, darg, 0); return 0;
     case IFL_STAGE_P2G: particles_to_grid(s); return 0;
     case IFL_STAGE_PARTICLE_ADVECT: particle_advect(s, darg); return 0;
     case IFL_STAGE_COPY_OLD: if (!q) return 1; qty_copy_old(q); return 0;

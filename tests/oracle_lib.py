@@ -217,6 +217,13 @@ class OracleSolver(_Base):
     def log(self):
         return self.lib.orc_log(self.h).decode()
 
+    def set_flip(self, alpha, cubic=False):
+        """Level 8: the FLIP update's alpha and its gridToParticles
+        interpolation (cubic: the Catmull-Rom variant, not the reference's)."""
+        f = self.lib.orc_set_flip
+        f.argtypes = [ctypes.c_void_p, ctypes.c_double, ctypes.c_int]
+        f(self.h, alpha, 1 if cubic else 0)
+
     def log_clear(self):
         self.lib.orc_log_clear(self.h)
 

@@ -292,3 +292,37 @@ def test_update_256():
               f"pressure {st.pressure.iterations}", flush=True)
     gpu.close()
     orc.close()
+
+
+def test_variant_alpha_and_cubic_g2p():
+    """The parameterised FLIP variant, not the reference: alpha 0.05 (a
+    "0.95 FLIP" blend, 8:1401) and the FLIP update's gridToParticles sampling
+    with the Catmull-Rom cerp of solvers 3-7 (7:404-422) instead of lerp
+    (flip_g2p = 1), against the oracle's restatement of the same variant, bit
+    for bit over 8 steps with bodies; and it really differs from the
+    reference's form after one step."""
+    import numpy as np
+    import ifl_fluids as F
+    sc = dict(ol.REF_SCENES[8])
+    sc["bodies"] = MULTI
+    F.frand_set_state(SEED0)
+    orc = ol.OracleSolver(8, 64, 64, sc["params"], sc["bodies"], fresh=True)
+    orc.set_flip(0.05, cubic=True)
+    gpu = make_product(8, 64, 64, sc["params"], sc["bodies"], dot_order=1, flip_alpha=0.05, flip_g2p=1)
+    for k in range(8):
+        orc.update(sc["dt"])
+        gpu.update(sc["dt"])
+        check(orc, gpu, ALL, f"variant step {k}")
+        if k == 0:
+            state = F.frand_state()
+            F.frand_set_state(SEED0)
+            std = make_product(8, 64, 64, sc["params"], sc["bodies"], dot_order=1)
+            std.update(sc["dt"])
+            assert not np.array_equal(std.field(ol.F_PROP_U), gpu.field(ol.F_PROP_U))
+            std.close()
+            F.frand_set_state(state)
+        if (k + 1) % sc["substeps"] == 0:
+            orc.bodies_update(sc["dt"])
+            gpu.bodies_update(sc["dt"])
+    gpu.close()
+    orc.close()
