@@ -114,16 +114,22 @@ class _OneCore:
         return False
 
     def host(self):
-        model = "unknown"
+        model, mhz = "unknown", None
         try:
             with open("/proc/cpuinfo") as f:
                 for line in f:
-                    if line.startswith("model name"):
+                    if line.startswith("model name") and model == "unknown":
                         model = line.split(":", 1)[1].strip()
-                        break
-        except OSError:
+                    elif line.startswith("cpu MHz") and mhz is None:
+                        mhz = float(line.split(":", 1)[1])
+        except (OSError, ValueError):
             pass
-        return {"cpu_model": model, "nproc": os.cpu_count(), "pinned_core": self.core}
+        try:  # the pinned core's current clock, where cpufreq reports it
+            with open(f"/sys/devices/system/cpu/cpu{self.core}/cpufreq/scaling_cur_freq") as f:
+                mhz = int(f.read()) / 1000.0
+        except (OSError, ValueError):
+            pass
+        return {"cpu_model": model, "cpu_mhz": mhz, "nproc": os.cpu_count(), "pinned_core": self.core}
 
 
 def cpu_baseline(scene, n, mean_iters):
