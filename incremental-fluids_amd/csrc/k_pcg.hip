@@ -1326,26 +1326,103 @@ namespace {
 // them (each block's partial is the single-solver kernel's)
 constexpr int kVecBlocks = 4;
 
+// vec_update_body (mode 0) over kVecBlocks blocks at once, every load of the
+// workgroup's cells issued before the first store; each cell's values are
+// vec_update_body's.
 __global__ __launch_bounds__(256) void batch_vec_update_kernel(const BatchInst* t, const int2* map) {
+    constexpr int K = kVecBlocks;
     const int2 m = map[blockIdx.x];
-    const VecArgs v = t[m.x].v;
+    const VecArgs a = t[m.x].v;
     const int nbv = t[m.x].nbv;
+    const PcgState* st = a.st;
+    if (st->done) return;
+    const long n = (long)a.sk.nstrips * a.sk.strip_sz;
+    bool on[K], fl[K];
+    long ci[K];
+    double sv[K], pv[K], zv[K], qv[K];
 #pragma unroll
-    for (int k = 0; k < kVecBlocks; k++) {
-        const int blk = m.y * kVecBlocks + k;
-        if (blk < nbv) vec_update_body(v, blk, 0, 1);
+    for (int k = 0; k < K; k++) {
+        const long blk = (long)m.y * K + k;
+        long i = blk * 256 + threadIdx.x;
+        int x, y;
+        on[k] = blk < nbv && i < n && skew_cell(a.sk, i, x, y);
+        if (!on[k]) i = 0;  // cell 0 stands in for the loads; nothing is stored
+        ci[k] = i;
+        fl[k] = a.flag[i] != 0;
+        sv[k] = a.s[i];
+        pv[k] = a.p[i];
+        zv[k] = a.z[i];
+        qv[k] = a.pm ? a.q[i] : 0.0;
+    }
+    const double alpha = st->alpha, beta = st->beta, zmean = st->zmean;
+#pragma unroll
+    for (int k = 0; k < K; k++) {
+        if (!on[k] || (a.vm && !fl[k])) continue;
+        a.p[ci[k]] = pv[k] + sv[k] * alpha;
+        double zi = a.pm && !fl[k] ? qv[k] : zv[k];
+        if (a.defl && fl[k]) zi = zi - zmean;
+        a.s[ci[k]] = zi + sv[k] * beta;
     }
 }
 
+// matvec_body<1> over kVecBlocks blocks at once: every load of the
+// workgroup's cells is issued before the first product. Each block's partial
+// is matvec_body's: the same terms, the same wave sums over the same lanes
+// and the same combination of the block's four wave sums.
 __global__ __launch_bounds__(256) void batch_matvec_kernel(const BatchInst* t, const int2* map) {
+    constexpr int K = kVecBlocks;
+    __shared__ double red[K][4];
     const int2 m = map[blockIdx.x];
-    const VecArgs v = t[m.x].v;
+    const VecArgs a = t[m.x].v;
     const int nbv = t[m.x].nbv;
-    for (int k = 0; k < kVecBlocks; k++) {
-        const int blk = m.y * kVecBlocks + k;
-        if (blk >= nbv) break;  // the same for the whole workgroup
-        matvec_body<1>(v, blk, 1);
-        __syncthreads();  // the block reduction's LDS is reused by the next block
+    if (a.st->done) return;
+    const Skew& sk = a.sk;
+    const long n = (long)sk.nstrips * sk.strip_sz;
+    const int W = sk.W, H = sk.H;
+    bool on[K], dt[K], hw[K], hs[K], he[K], hn[K];
+    long ci[K];
+    double b0[K], bw[K], bs[K], be[K], bn[K], ad[K], axw[K], ays[K], ax[K], ay[K];
+#pragma unroll
+    for (int k = 0; k < K; k++) {
+        const long blk = (long)m.y * K + k;
+        long i = blk * 256 + threadIdx.x;
+        int x = 0, y = 0;
+        on[k] = blk < nbv && i < n && skew_cell(sk, i, x, y);
+        if (!on[k]) i = 0;  // cell 0 stands in for the loads; nothing is kept
+        ci[k] = i;
+        const int l = y & 63;
+        hw[k] = on[k] && x > 0;
+        hs[k] = on[k] && y > 0;
+        he[k] = on[k] && x < W - 1;
+        hn[k] = on[k] && y < H - 1;
+        // a missing neighbour reads cell i and is not added
+        const long jw = hw[k] ? i - kStrip : i;
+        const long js = hs[k] ? (l > 0 ? i - kStrip - 1 : sk.pos(x, y - 1)) : i;
+        const long je = he[k] ? i + kStrip : i;
+        const long jn = hn[k] ? (l < kStrip - 1 ? i + kStrip + 1 : sk.pos(x, y + 1)) : i;
+        b0[k] = a.s[i]; bw[k] = a.s[jw]; bs[k] = a.s[js]; be[k] = a.s[je]; bn[k] = a.s[jn];
+        ad[k] = a.adiag[i]; axw[k] = a.apx[jw]; ays[k] = a.apy[js]; ax[k] = a.apx[i]; ay[k] = a.apy[i];
+        // matvec_body's dot term: every cell unless the vector operations are masked
+        dt[k] = on[k] && (!a.vm || a.flag[i]);
+    }
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+#pragma unroll
+    for (int k = 0; k < K; k++) {
+        double tv = ad[k] * b0[k];
+        tv = hw[k] ? tv + axw[k] * bw[k] : tv;
+        tv = hs[k] ? tv + ays[k] * bs[k] : tv;
+        tv = he[k] ? tv + ax[k] * be[k] : tv;
+        tv = hn[k] ? tv + ay[k] * bn[k] : tv;
+        if (on[k]) a.q[ci[k]] = tv;
+        double prod = dt[k] ? tv * b0[k] : 0.0;
+        prod = wave_sum(prod);
+        if (lane == 0) red[k][w] = prod;
+    }
+    __syncthreads();
+    if (threadIdx.x < K) {
+        const int k = threadIdx.x;
+        const long blk = (long)m.y * K + k;
+        if (blk < nbv) a.partial[blk] = (red[k][0] + red[k][1]) + (red[k][2] + red[k][3]);
     }
 }
 
