@@ -1567,14 +1567,17 @@ int batch_build(Solver* const* ss, int n, BatchDev* d) {
         for (int k = 0; k * kVecBlocks < b.nbv; k++) d->vmap4_h.push_back(make_int2(i, k));
         sts[i] = s->st;
     }
-    // Strips are dealt by their distance from the sweep's start (every
-    // instance's first strip, then every second strip, ...), larger
-    // instances first within a rank. A strip's upstream neighbour thus has a
-    // lower block id, so the strips need not be resident together; and a
-    // strip is dispatched after the previous rank, so it seldom holds a CU
-    // while it waits for its upstream neighbour (instance-major order, all
-    // strips of an instance together: 230 / 206 us per forward / backward
-    // launch on the 256-instance draw; this order: 184 / 173 us).
+    // Strips are dealt by their distance from the end of their instance's
+    // sweep: every instance's last strip at rank `most - 1`, its first at
+    // `most - nstrips` (rank by rank, larger instances first within a rank).
+    // A strip's upstream neighbour thus has a lower block id, so the strips
+    // need not be resident together; a strip is dispatched after the previous
+    // rank, so it seldom holds a CU while it waits for its upstream
+    // neighbour; and the largest instances, whose chains of strips are the
+    // longest, start first. Forward / backward us per launch on the
+    // 256-instance draw: instance-major order 230 / 206; ranked from each
+    // sweep's start 182 / 170; this order 176 / 164 (ranks weighted further
+    // toward the large instances: 182-192 / 168-176).
     {
         std::vector<int> order(n);
         for (int i = 0; i < n; i++) order[i] = i;
@@ -1586,9 +1589,10 @@ int batch_build(Solver* const* ss, int n, BatchDev* d) {
         for (int r = 0; r < most; r++)
             for (int i : order) {
                 const int ns = ss[i]->sk.nstrips;
-                if (r >= ns) continue;
-                fmap.push_back(make_int2(i, r));
-                bmap.push_back(make_int2(i, ns - 1 - r));
+                const int k = r - (most - ns);  // the strip of instance i at this rank
+                if (k < 0) continue;
+                fmap.push_back(make_int2(i, k));
+                bmap.push_back(make_int2(i, ns - 1 - k));
             }
     }
     Solver* s = s0;  // error reporting for the uploads
