@@ -533,7 +533,10 @@ def measure_batch(n_inst, steps, warmup, mode="fused", threads=16, level=4):
             p = batch.profile_read(kid)
             if p["launches"] == 0 or p["ms"] <= 0:
                 continue
-            byts = (BATCH_BYTES_PER_CELL if level == 4 else KERNEL_BYTES_PER_CELL)[name] * p["cell_iterations"]
+            # levels 6-8 fold the vector update into the batch's product too
+            bpc = (FUSED_MATVEC_BYTES_PER_CELL if name == "matvec" and level >= 6
+                   else (BATCH_BYTES_PER_CELL if level == 4 else KERNEL_BYTES_PER_CELL)[name])
+            byts = bpc * p["cell_iterations"]
             kern[name] = {"launches": p["launches"], "total_ms": p["ms"], "avg_us": 1e3 * p["ms"] / p["launches"],
                           "cell_iterations": p["cell_iterations"], "bytes": byts,
                           "achieved_GBps": byts / (p["ms"] * 1e-3) / 1e9}

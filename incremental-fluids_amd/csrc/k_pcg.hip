@@ -159,14 +159,12 @@ __global__ __launch_bounds__(256) void matvec_kernel(VecArgs a, int check_done) 
 // forms it (z + s beta at a FLUID cell, s unchanged elsewhere), so the terms
 // are vec_update_kernel's and matvec_kernel's values; the new s goes to a
 // second array, since neighbouring workgroups still read the old one.
-__global__ __launch_bounds__(256) void matvec_update_kernel(VecArgs a, const double* __restrict__ so,
-                                                            double* __restrict__ sn) {
+IFL_DEV void matvec_update_body(const VecArgs& a, long blk, const double* __restrict__ so, double* __restrict__ sn) {
     __shared__ double red[4];
     PcgState* st = a.st;
     if (st->done) return;
     const Skew& sk = a.sk;
     const long n = (long)sk.nstrips * sk.strip_sz;
-    const long blk = blockIdx.x;
     const long i = blk * (long)blockDim.x + threadIdx.x;
     const double alpha = st->alpha, beta = st->beta;
     double prod = 0.0;
@@ -207,6 +205,10 @@ __global__ __launch_bounds__(256) void matvec_update_kernel(VecArgs a, const dou
     if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = prod;
     __syncthreads();
     if (threadIdx.x == 0) a.partial[blk] = (red[0] + red[1]) + (red[2] + red[3]);
+}
+__global__ __launch_bounds__(256) void matvec_update_kernel(VecArgs a, const double* __restrict__ so,
+                                                            double* __restrict__ sn) {
+    matvec_update_body(a, blockIdx.x, so, sn);
 }
 
 // Fixed-order sum of `n` partials by one workgroup of 256 threads.
@@ -1267,7 +1269,18 @@ struct BatchInst {
     double tol;
     int limit;
     int pad;
+    double* s2;  // the second s when the batch folds the vector update into the product, else null
 };
+
+// The array holding an instance's s after iteration `it` (s and s2 by turns
+// when the batch folds the update into the product, as for one solver).
+IFL_DEV double* batch_s_after(const BatchInst& b, int it) { return b.s2 && it > 0 && (it & 1) ? b.s2 : b.v.s; }
+// The array holding it at the end of a solve: where the last product read it.
+IFL_DEV double* batch_s_last(const BatchInst& b) {
+    const PcgState* st = b.v.st;
+    const int worked = st->status == IFL_SOLVE_CONVERGED ? st->printed_iter + 1 : st->iter;
+    return batch_s_after(b, worked - 1);
+}
 
 struct BatchDev {
     int n = 0, nvblk = 0, nsblk = 0;
@@ -1297,6 +1310,7 @@ struct BatchDev {
     int nvblk4 = 0;
     int nvblk_full = 0, nsblk_full = 0;
     bool compacted = false;
+    bool fused = false;  // the vector update folded into the product (every instance: levels 6-8 with s2)
     // profiling: events around the fused launches (matvec, forward and
     // backward sweep) of the current solve
     std::vector<hipEvent_t> ev_pool;
@@ -1426,6 +1440,19 @@ __global__ __launch_bounds__(256) void batch_matvec_kernel(const BatchInst* t, c
     }
 }
 
+// matvec_update_body for every instance: the vector update of iteration
+// it - 1 and the product of iteration it in one pass (levels 6-8), one block
+// of an instance per workgroup; s is read from the array iteration it - 1
+// left it in and the new s goes to the other. (Four blocks per workgroup, the
+// block reduction's LDS reused between them: 25.3 against 23.7 us per launch
+// on 16 FLIP instances at 256^2.)
+__global__ __launch_bounds__(256) void batch_matvec_update_kernel(const BatchInst* t, const int2* map, int it) {
+    const int2 m = map[blockIdx.x];
+    const BatchInst& b = t[m.x];
+    const VecArgs a = b.v;
+    matvec_update_body(a, m.y, batch_s_after(b, it - 1), batch_s_after(b, it));
+}
+
 __global__ __launch_bounds__(256) void batch_fin_alpha_kernel(const BatchInst* t) {
     fin_alpha_body(t[blockIdx.x].v, t[blockIdx.x].nbv, 0);
 }
@@ -1486,7 +1513,8 @@ __global__ __launch_bounds__(256) void batch_fin_sigma_init_kernel(const BatchIn
 __global__ __launch_bounds__(256) void batch_tail_kernel(const BatchInst* t, const int2* map) {
     const int2 m = map[blockIdx.x];
     const BatchInst& b = t[m.x];
-    const VecArgs& a = b.v;
+    VecArgs a = b.v;
+    a.s = batch_s_last(b);  // with the folded update: the tails work on that array, and it ends in s
     const int status = a.st->status;
     const long n = (long)a.sk.nstrips * a.sk.strip_sz;
     const long i = (long)m.y * blockDim.x + threadIdx.x;
@@ -1498,6 +1526,7 @@ __global__ __launch_bounds__(256) void batch_tail_kernel(const BatchInst* t, con
         int x, y;
         if (a.pm && i < n && skew_cell(a.sk, i, x, y) && !a.flag[i]) a.z[i] = a.q[i];
     }
+    if (a.s != b.v.s && i < n) b.v.s[i] = a.s[i];  // the same thread wrote it above
 }
 
 __global__ void batch_gather_kernel(PcgState* const* st, int n, PcgState* out) {
@@ -1547,6 +1576,7 @@ int batch_build(Solver* const* ss, int n, BatchDev* d) {
         b.tol = s->cfg.tol;
         b.limit = s->cfg.pcg_limit;
         b.pad = 0;
+        b.s2 = nullptr;
         fwd[i] = fwd_args<1, double>(s);
         bwd[i] = bwd_args<1, double>(s);
         size_t shf = 0, shb = 0;
@@ -1602,6 +1632,11 @@ int batch_build(Solver* const* ss, int n, BatchDev* d) {
     d->vmap_h = vmap;
     d->fmap_h = fmap;
     d->bmap_h = bmap;
+    // the vector update folds into the product when every instance can, as
+    // for one solver (fused_update: the second s, tree dots, masked vectors)
+    d->fused = true;
+    for (int i = 0; i < n; i++) d->fused = d->fused && fused_update(ss[i], inst[i].v);
+    for (int i = 0; i < n; i++) inst[i].s2 = d->fused ? ss[i]->s2 : nullptr;
     IFL_TRY(upload(s, &d->inst, inst));
     IFL_TRY(upload(s, &d->fwd, fwd));
     IFL_TRY(upload(s, &d->bwd, bwd));
@@ -1651,12 +1686,16 @@ int batch_time(Solver* s, BatchDev* d, hipStream_t stream, int kernel, bool begi
 
 template <int LV, int NGF, int NGB>
 int batch_iteration(Solver* s, BatchDev* d, hipStream_t stream, int it, bool prof) {
-    if (it > 0) {
+    const bool fuse = it > 0 && d->fused;
+    if (it > 0 && !fuse) {
         batch_vec_update_kernel<<<d->nvblk4, 256, 0, stream>>>(d->inst, d->vmap4);
         IFL_LAUNCH_CHECK();
     }
     if (prof) IFL_TRY(batch_time(s, d, stream, IFL_KERNEL_MATVEC, true));
-    batch_matvec_kernel<<<d->nvblk4, 256, 0, stream>>>(d->inst, d->vmap4);
+    if (fuse)
+        batch_matvec_update_kernel<<<d->nvblk, 256, 0, stream>>>(d->inst, d->vmap, it);
+    else
+        batch_matvec_kernel<<<d->nvblk4, 256, 0, stream>>>(d->inst, d->vmap4);
     IFL_LAUNCH_CHECK();
     if (prof) IFL_TRY(batch_time(s, d, stream, IFL_KERNEL_MATVEC, false));
     batch_fin_alpha_kernel<<<d->n, 256, 0, stream>>>(d->inst);
@@ -1860,6 +1899,14 @@ int k_project_batch(Solver* const* ss, int n, hipStream_t stream, BatchDev** dev
         long nn = skew_n(si);
         s = si;
         IFL_HIP(hipStreamWaitEvent(si->stream, d->joined, 0));
+        // with the folded update, s is where the last product read it; the
+        // tails work on that array, and it ends in s
+        double* s_last = nullptr;
+        if (d->fused) {
+            const int worked = h.status == IFL_SOLVE_CONVERGED ? h.printed_iter + 1 : h.iter;
+            s_last = s_after(si, worked - 1);
+            a.s = s_last;
+        }
         if (h.status == IFL_SOLVE_CONVERGED) {
             vec_update_kernel<<<grid_blocks(nn, 256), 256, 0, si->stream>>>(a, 2, 0);  // p += alpha*s (7:869)
             IFL_LAUNCH_CHECK();
@@ -1872,6 +1919,8 @@ int k_project_batch(Solver* const* ss, int n, hipStream_t stream, BatchDev** dev
                 IFL_LAUNCH_CHECK();
             }
         }
+        if (s_last && s_last != si->s)
+            IFL_HIP(hipMemcpyAsync(si->s, s_last, nn * sizeof(double), hipMemcpyDeviceToDevice, si->stream));
         if (stats) {
             stats[i].status = h.status;
             stats[i].iterations = h.printed_iter;

@@ -328,3 +328,75 @@ def test_fused_batch_host_bodies(level):
         single.close()
         batched[k].close()
     print(f"level {level}: {rows_b}", flush=True)
+
+
+@pytest.mark.parametrize("level,limit", [(7, 7), (7, 8), (8, 7), (8, 8)])
+def test_fused_batch_budget_exits(level, limit):
+    """With a PCG budget the pressure solves end out of budget after an odd or
+    an even number of iterations. At levels 6-8 the batch folds the vector
+    update into the product (s and a second array by turns, 7:869 and 7:881
+    in one pass), so an instance's last s lies in either array: its tail
+    (the deferred s update, 7:881) must work on that array and leave the
+    result in s, as a solver stepped alone does. Level 7 takes the batched
+    tail, level 8 the tails on the solvers' own streams."""
+    import ifl_fluids as F
+    import ifl_scenes as S
+    n, steps = 4, 2
+    if level == 8:
+        sc = S.config5_scene()
+        dt = sc["dt"]
+
+        def make(k):
+            return F.FluidSolver(8, 128, 128, sc["params"], [F.SolidBox(*b[1:]) for b in sc["bodies"]],
+                                 pcg_limit=limit)
+
+        def inflows(k):
+            return []  # solver 8 adds its inflow inside update()
+    else:
+        insts = [i for i in S.config4_instances(64) if max(i["w"], i["h"]) <= 160][:n]
+        dt = insts[0]["dt"]
+
+        def make(k):
+            i = insts[k]
+            bodies = [F.SolidBox(*(tuple(b[1:6]) + (0.0, 0.0, 0.0))) for b in i["bodies"]]
+            return F.FluidSolver(7, i["w"], i["h"], (0.1, 1.0, 0.01), bodies, pcg_limit=limit)
+
+        def inflows(k):
+            # d = 1, t = 600, (u, v) = (0, 3): every pressure solve has work to do
+            return [insts[k]["inflows"][0][:5] + (600.0, 0.0, 3.0)]
+    seed = 0x5EED1234
+    F.frand_set_state(seed)
+    batched = [make(k) for k in range(n)]
+    batch = F.Batch(batched)
+    rows_b = []
+    for _ in range(steps):
+        entries = [(k,) + tuple(e) for k in range(n) for e in inflows(k)]
+        if entries:
+            batch.add_inflow(entries)
+        st = batch.update(dt)
+        rows_b.append([(x.heat.status, x.heat.iterations, x.pressure.status, x.pressure.iterations)
+                       for x in st])
+    batch.close()
+    F.frand_set_state(seed)
+    singles = [make(k) for k in range(n)]
+    rows_s = []
+    for _ in range(steps):
+        row = []
+        for k, s in enumerate(singles):
+            for e in inflows(k):
+                s.add_inflow(*e)
+            x = s.update(dt)
+            row.append((x.heat.status, x.heat.iterations, x.pressure.status, x.pressure.iterations))
+        rows_s.append(row)
+    print(f"level {level} limit {limit}: {rows_b}", flush=True)
+    assert rows_s == rows_b
+    # the budget exits this test is about (some drawn instances' inflows
+    # leave their pressure solves nothing to do)
+    budget = [r[2] == F.SOLVE_BUDGET for row in rows_b for r in row]
+    assert sum(budget) >= len(budget) // 2
+    fields = ol.state_fields(level)
+    for k in range(n):
+        bad = compare_fields(singles[k], batched[k], fields, where=f"level {level} instance {k}")
+        assert not bad, "\n".join(bad[:6])
+        singles[k].close()
+        batched[k].close()
